@@ -1,0 +1,175 @@
+// Contractions of the sweep as MFMA GEMMs (see contract.hpp).
+#include "contract.hpp"
+
+namespace dmrg {
+
+namespace {
+inline GemmTile pick(int M, int N, int K) {
+  return gemm_pick_tile(M, N, K, false);
+}
+template <class T> inline GemmTile pickT(int M, int N, int K) {
+  return gemm_pick_tile(M, N, K, is_cplx<T>::value);
+}
+}  // namespace
+
+template <class T>
+void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, const T* theta, T* out,
+                 int cL, int cR, int d, int D, ContractScratch<T>& sc) {
+  const int dd = d * d;
+  const size_t need = (size_t)D * dd * cL * cR;
+  DMRG_REQUIRE(need <= sc.cap, "heff2: scratch too small");
+  const T one(1), zero(0);
+  // X[(w + D a'), (ss + dd b)] = sum_a L[a, w, a'] theta[a, ss, b]
+  gemm<T>(st, OP_T, OP_N, D * cL, dd * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
+          pickT<T>(D * cL, dd * cR, cL));
+  // Y[a' + cL ss' + cL dd b + cL dd cR n] = sum_{w,ss} X * WW[(w,ss),(ss',n)]
+  View2 vx{{cL, (long long)D, (long long)D * cL * dd}, {D, 1, (long long)D * cL}};
+  View2 vw = view_colmajor(D * dd);
+  View2 vy{{cL, 1, (long long)cL * dd}, {dd, (long long)cL, (long long)cL * dd * cR}};
+  gemm_tensor<T>(st, cL * cR, dd * D, D * dd, one, sc.X, vx, false, false, WW, vw, false, true,
+                 zero, sc.Y, vy, pickT<T>(cL * cR, dd * D, D * dd));
+  // out[(a' + cL ss'), b'] = sum_{(b, n)} Y * R[(b + cR n), b']
+  gemm<T>(st, OP_N, OP_N, cL * dd, cR, D * cR, one, sc.Y, cL * dd, Renv, cR * D, zero, out,
+          cL * dd, pickT<T>(cL * dd, cR, D * cR));
+}
+
+template <class T>
+void heff1_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* W, const T* theta, T* out,
+                 int cL, int cR, int d, int D, ContractScratch<T>& sc) {
+  const size_t need = (size_t)D * d * cL * cR;
+  DMRG_REQUIRE(need <= sc.cap, "heff1: scratch too small");
+  const T one(1), zero(0);
+  gemm<T>(st, OP_T, OP_N, D * cL, d * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
+          pickT<T>(D * cL, d * cR, cL));
+  View2 vx{{cL, (long long)D, (long long)D * cL * d}, {D, 1, (long long)D * cL}};
+  View2 vw = view_colmajor(D * d);
+  View2 vy{{cL, 1, (long long)cL * d}, {d, (long long)cL, (long long)cL * d * cR}};
+  gemm_tensor<T>(st, cL * cR, d * D, D * d, one, sc.X, vx, false, false, W, vw, false, true, zero,
+                 sc.Y, vy, pickT<T>(cL * cR, d * D, D * d));
+  gemm<T>(st, OP_N, OP_N, cL * d, cR, D * cR, one, sc.Y, cL * d, Renv, cR * D, zero, out, cL * d,
+          pickT<T>(cL * d, cR, D * cR));
+}
+
+template <class T>
+void env_left(hipStream_t st, const T* Lenv, const T* A, const T* W, T* Lnew, int ci, int co, int d,
+              int D, ContractScratch<T>& sc) {
+  const size_t need = (size_t)D * d * ci * co;
+  DMRG_REQUIRE(need <= sc.cap, "env_left: scratch too small");
+  const T one(1), zero(0);
+  gemm<T>(st, OP_T, OP_N, D * ci, d * co, ci, one, Lenv, ci, A, ci, zero, sc.X, D * ci,
+          pickT<T>(D * ci, d * co, ci));
+  View2 vx{{ci, (long long)D, (long long)D * ci * d}, {D, 1, (long long)D * ci}};
+  View2 vw = view_colmajor(D * d);
+  View2 vy{{ci, 1, (long long)ci * d}, {d, (long long)ci, (long long)ci * d * co}};
+  gemm_tensor<T>(st, ci * co, d * D, D * d, one, sc.X, vx, false, false, W, vw, false, true, zero,
+                 sc.Y, vy, pickT<T>(ci * co, d * D, D * d));
+  gemm<T>(st, OP_T, OP_C, co * D, co, ci * d, one, sc.Y, ci * d, A, ci * d, zero, Lnew, co * D,
+          pickT<T>(co * D, co, ci * d));
+}
+
+template <class T>
+void env_right(hipStream_t st, const T* Renv, const T* A, const T* Wr, T* Rnew, int ci, int co,
+               int d, int D, ContractScratch<T>& sc) {
+  const size_t need = (size_t)D * d * ci * co;
+  DMRG_REQUIRE(need <= sc.cap, "env_right: scratch too small");
+  const T one(1), zero(0);
+  // X[(a + co s), (w' + D b')] = sum_b A[a, s, b] R[b, w', b']
+  gemm<T>(st, OP_N, OP_N, co * d, D * ci, ci, one, A, co * d, Renv, ci, zero, sc.X, co * d,
+          pickT<T>(co * d, D * ci, ci));
+  // Y[a + co w + co D sp + co D d b'] = sum_{s, w'} X * Wr[(s, w'), (w, sp)]
+  View2 vx{{co, 1, (long long)co * d * D}, {d, (long long)co, (long long)co * d}};
+  View2 vw = view_colmajor(d * D);
+  View2 vy{{co, 1, (long long)co * D * d}, {D, (long long)co, (long long)co * D}};
+  gemm_tensor<T>(st, co * ci, D * d, d * D, one, sc.X, vx, false, false, Wr, vw, false, true, zero,
+                 sc.Y, vy, pickT<T>(co * ci, D * d, d * D));
+  // Rnew[(a + co w), a'] = sum_{(sp, b')} Y * conj(A[a', sp, b'])
+  gemm<T>(st, OP_N, OP_H, co * D, co, d * ci, one, sc.Y, co * D, A, co, zero, Rnew, co * D,
+          pickT<T>(co * D, co, d * ci));
+}
+
+template <class T>
+void build_ww2(const T* W1, const T* W2, int d, int D, T* WW) {
+  const int dd = d * d, K = D * dd;
+  for (int i = 0; i < K * K; ++i) WW[i] = T(0);
+  auto w = [&](const T* W, int a, int s, int sp, int b) {
+    return W[a + (size_t)D * (s + (size_t)d * (sp + (size_t)d * b))];
+  };
+  for (int a = 0; a < D; ++a)
+    for (int n = 0; n < D; ++n)
+      for (int s1 = 0; s1 < d; ++s1)
+        for (int s2 = 0; s2 < d; ++s2)
+          for (int s1p = 0; s1p < d; ++s1p)
+            for (int s2p = 0; s2p < d; ++s2p) {
+              T v(0);
+              for (int m = 0; m < D; ++m) v = v + w(W1, a, s1, s1p, m) * w(W2, m, s2, s2p, n);
+              int row = a + D * (s1 + d * s2);
+              int col = (s1p + d * s2p) + dd * n;
+              WW[row + (size_t)K * col] = v;
+            }
+}
+
+template <class T>
+void build_wright(const T* W, int d, int D, T* Wr) {
+  for (int w = 0; w < D; ++w)
+    for (int s = 0; s < d; ++s)
+      for (int sp = 0; sp < d; ++sp)
+        for (int wp = 0; wp < D; ++wp)
+          Wr[(s + d * wp) + (size_t)d * D * (w + D * sp)] =
+              W[w + (size_t)D * (s + (size_t)d * (sp + (size_t)d * wp))];
+}
+
+namespace {
+__global__ void k_f64_f32(const double* in, float* out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
+}
+__global__ void k_f32_f64(const float* in, double* out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (double)in[i];
+}
+template <class T>
+__global__ void k_unit(T* env, int D, int w) {
+  int i = threadIdx.x;
+  if (i < D) env[i] = (i == w) ? T(1) : T(0);
+}
+}  // namespace
+
+void convert_f64_f32(hipStream_t st, const double* in, float* out, size_t n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_f64_f32, dim3(std::min<size_t>(cdiv(n, 256), 4096)), dim3(256), 0, st, in,
+                     out, n);
+  DMRG_LAUNCH_CHECK();
+}
+void convert_f32_f64(hipStream_t st, const float* in, double* out, size_t n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_f32_f64, dim3(std::min<size_t>(cdiv(n, 256), 4096)), dim3(256), 0, st, in,
+                     out, n);
+  DMRG_LAUNCH_CHECK();
+}
+template <class T>
+void set_boundary_env(hipStream_t st, T* env, int D, int w) {
+  hipLaunchKernelGGL(k_unit<T>, dim3(1), dim3(64), 0, st, env, D, w);
+  DMRG_LAUNCH_CHECK();
+}
+
+#define DMRG_INST(T)                                                                          \
+  template void heff2_apply<T>(hipStream_t, const T*, const T*, const T*, const T*, T*, int,  \
+                               int, int, int, ContractScratch<T>&);                           \
+  template void heff1_apply<T>(hipStream_t, const T*, const T*, const T*, const T*, T*, int,  \
+                               int, int, int, ContractScratch<T>&);                           \
+  template void env_left<T>(hipStream_t, const T*, const T*, const T*, T*, int, int, int, int, \
+                            ContractScratch<T>&);                                             \
+  template void env_right<T>(hipStream_t, const T*, const T*, const T*, T*, int, int, int,    \
+                             int, ContractScratch<T>&);                                       \
+  template void build_ww2<T>(const T*, const T*, int, int, T*);                               \
+  template void build_wright<T>(const T*, int, int, T*);
+DMRG_INST(double)
+DMRG_INST(cdouble)
+DMRG_INST(float)
+#undef DMRG_INST
+template void set_boundary_env<double>(hipStream_t, double*, int, int);
+template void set_boundary_env<cdouble>(hipStream_t, cdouble*, int, int);
+
+}  // namespace dmrg

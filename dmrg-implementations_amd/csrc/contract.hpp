@@ -1,0 +1,61 @@
+// Tensor-network contractions of the sweep, written as MFMA GEMMs.
+//
+// Layouts (column-major, as in the reference GPU engines):
+//   MPS     A[a + s*cL + b*cL*d]                       (cL, d, cR)
+//   theta   th[a + s1*cL + s2*cL*d + b*cL*d*d]          (cL, d, d, cR)
+//   env     L[a + w*chi + a2*chi*D]   (ket, mpo, bra)   (chi, D, chi)
+//   MPO     W[w + s*D + sp*D*d + wp*D*d*d]  (s ket, sp bra)
+// (gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h:582-934, test_dmrg2_gpu.cpp:22-34)
+//
+// Every contraction is three launches: a big GEMM against an environment, an
+// MPO-bond GEMM with two-level addressing (K = D*d or D*d^2), and a big GEMM
+// against the other environment / MPS.  No permutation passes.
+#pragma once
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace dmrg {
+
+// Scratch for one contraction stream: X and Y each hold D*d^2*chi^2 entries.
+template <class T>
+struct ContractScratch {
+  T* X = nullptr;
+  T* Y = nullptr;
+  size_t cap = 0;
+};
+
+// Two-site H_eff: out = H_eff(theta); WW is the fused two-site MPO as a
+// (D*d^2) x (D*d^2) matrix WW[(w + D*(s1 + d*s2)) + D*d^2*((s1p + d*s2p) + d^2*n)].
+template <class T>
+void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, const T* theta, T* out,
+                 int cL, int cR, int d, int D, ContractScratch<T>& sc);
+
+// Single-site H_eff (dmrg_gpu_impl.h:538-705); W is the raw MPO tensor.
+template <class T>
+void heff1_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* W, const T* theta, T* out,
+                 int cL, int cR, int d, int D, ContractScratch<T>& sc);
+
+// Lnew (co, D, co) from L (ci, D, ci), A (ci, d, co), W (raw).
+template <class T>
+void env_left(hipStream_t st, const T* Lenv, const T* A, const T* W, T* Lnew, int ci, int co, int d,
+              int D, ContractScratch<T>& sc);
+
+// Rnew (co, D, co) from R (ci, D, ci), A (co, d, ci), Wr (right-env MPO matrix
+// Wr[(s + d*wp) + d*D*(w + D*sp)] = W[w, s, sp, wp]).
+template <class T>
+void env_right(hipStream_t st, const T* Renv, const T* A, const T* Wr, T* Rnew, int ci, int co,
+               int d, int D, ContractScratch<T>& sc);
+
+// Host builders of the derived MPO matrices (from the raw layout).
+template <class T>
+void build_ww2(const T* W1, const T* W2, int d, int D, T* WW);
+template <class T>
+void build_wright(const T* W, int d, int D, T* Wr);
+
+// Elementwise helpers.
+void convert_f64_f32(hipStream_t st, const double* in, float* out, size_t n);
+void convert_f32_f64(hipStream_t st, const float* in, double* out, size_t n);
+template <class T>
+void set_boundary_env(hipStream_t st, T* env, int D, int w);  // (1, D, 1) unit vector e_w
+
+}  // namespace dmrg

@@ -1,0 +1,645 @@
+// DMRG sweep engine: sweep driver, Lanczos driver, truncation split.
+// Follows gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h (two-site) and
+// gpu-rocm/dmrg-gpu/src/dmrg_gpu_impl.h (single-site) step by step; the
+// library calls there are replaced by the kernels of gemm/contract/lanczos/svd.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dmrg {
+
+namespace {
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+template <class T> bool finite_v(T v) { return std::isfinite(real(v)) && std::isfinite(imag(v)); }
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Lanczos driver (dmrg2_gpu_impl.h:970-1160)
+// ---------------------------------------------------------------------------
+template <class T>
+double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int site,
+                             const std::function<void(const T*, T*)>& matvec) {
+  const int max_iter = (int)std::min<size_t>((size_t)kmax, n);
+  DMRG_REQUIRE(max_iter >= 1, "lanczos: empty problem");
+  if ((int)ev.size() < 2 * max_iter) {
+    size_t old = ev.size();
+    ev.resize(2 * max_iter);
+    for (size_t i = old; i < ev.size(); ++i) DMRG_HIP(hipEventCreate(&ev[i]));
+  }
+  lz_reset(st, *lz);
+  // v0 = theta/|theta|; zero-norm start replaced by the seeded stream 42+site
+  lz_norm<T>(st, *lz, theta, n);
+  lz_fill_if_small<T>(st, *lz, theta, n, (uint64_t)(42 + site), 1e-14);
+  lz_norm<T>(st, *lz, theta, n);
+  lz_scale<T>(st, *lz, V, theta, n);
+  int niter = max_iter, it = 0, nmv = 0;
+  for (it = 0; it < max_iter; ++it) {
+    T* vi = V + (size_t)it * n;
+    DMRG_HIP(hipEventRecord(ev[2 * it], st));
+    matvec(vi, work);
+    DMRG_HIP(hipEventRecord(ev[2 * it + 1], st));
+    ++nmv;
+    lz_alpha<T>(st, *lz, vi, work, n, it);
+    lz_sub_dots<T>(st, *lz, work, V, n, it);
+    lz_orth_norm<T>(st, *lz, work, V, n, it);
+    if (it + 1 < max_iter) lz_scale<T>(st, *lz, V + (size_t)(it + 1) * n, work, n);
+    if (it >= 4 && it % 3 == 0) {
+      lz_check(st, *lz, it + 1, beta_tol, eig_tol);
+      DMRG_HIP(hipMemcpyAsync(h_status, lz->status, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+      DMRG_HIP(hipStreamSynchronize(st));
+      if (h_status[0]) {
+        niter = h_status[1];
+        break;
+      }
+    }
+  }
+  lz_final(st, *lz, niter);
+  lz_ritz<T>(st, *lz, theta, V, n, niter);
+  lz_scale<T>(st, *lz, theta, theta, n);
+  DMRG_HIP(hipMemcpyAsync(h_scal, lz->scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+  DMRG_HIP(hipStreamSynchronize(st));
+  double ms = 0.0;
+  for (int j = 0; j < nmv; ++j) {
+    float t = 0.f;
+    DMRG_HIP(hipEventElapsedTime(&t, ev[2 * j], ev[2 * j + 1]));
+    ms += t;
+  }
+  last_heff_ms = ms;
+  last_niter = niter;
+  last_matvecs = nmv;
+  const double E = h_scal[3];
+  if (!std::isfinite(E) || !(h_scal[0] > 0.0))
+    throw Error(ST_NONFINITE, "lanczos: non-finite energy or zero Ritz vector at site " +
+                                  std::to_string(site));
+  return E;
+}
+
+// ---------------------------------------------------------------------------
+// Construction (dmrg2_gpu_impl.h:51-227)
+// ---------------------------------------------------------------------------
+template <class T>
+Engine<T>::Engine(int L_, int d_, int chi_, int D_, double tol_, int variant_, const Options& o)
+    : L(L_), d(d_), chi(chi_), D(D_), tol(tol_), variant(variant_), opt(o) {
+  DMRG_REQUIRE(L >= 2, "L must be >= 2");
+  DMRG_REQUIRE(d >= 1 && d <= 64, "bad physical dimension");
+  DMRG_REQUIRE(chi >= 1 && D >= 1, "bad chi_max / D");
+  DMRG_REQUIRE(variant == 1 || variant == 2, "variant must be 1 or 2");
+  DMRG_REQUIRE(opt.krylov_max >= 1 && opt.krylov_max <= 128, "krylov_max must be in [1, 128]");
+  DMRG_REQUIRE(opt.precision == 0 || (opt.precision == 1 && !is_cplx<T>::value && variant == 2),
+               "FP32 matvec mode is available for the real two-site engine");
+  DMRG_REQUIRE((long long)chi * d <= 2048, "chi_max * d must be <= 2048");
+  DMRG_HIP(hipSetDevice(opt.device));
+  bond.assign(L + 1, 1);
+  for (int i = 1; i < L; ++i) {
+    int e = std::min(i, L - i);
+    double ex = std::pow((double)d, (double)e);
+    bond[i] = ex > chi ? chi : (int)ex;
+  }
+  try {
+    alloc_all();
+  } catch (...) {
+    free_all();
+    throw;
+  }
+}
+
+template <class T>
+Engine<T>::~Engine() {
+  free_all();
+}
+
+template <class T>
+void Engine<T>::alloc_all() {
+  DMRG_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
+  DMRG_HIP(hipStreamCreateWithFlags(&s_env, hipStreamNonBlocking));
+  DMRG_HIP(hipEventCreateWithFlags(&ev_canon, hipEventDisableTiming));
+  DMRG_HIP(hipEventCreateWithFlags(&ev_env, hipEventDisableTiming));
+  const size_t site_sz = (size_t)chi * d * chi;
+  const size_t env_sz = (size_t)chi * D * chi;
+  const int dv = (variant == 2) ? d * d : d;
+  nmax = (size_t)chi * dv * chi;
+  kmax = std::min<size_t>((size_t)opt.krylov_max, nmax);
+  auto dmalloc = [](void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(ST_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+  };
+  mps.assign(L, nullptr);
+  for (int i = 0; i < L; ++i) dmalloc((void**)&mps[i], site_sz * sizeof(T));
+  dmalloc((void**)&mps_spare, site_sz * sizeof(T));
+  Lenv.assign(L + 1, nullptr);
+  Renv.assign(L + 1, nullptr);
+  for (int i = 0; i <= L; ++i) {
+    size_t sz = (i == 0 || i == L) ? (size_t)D : env_sz;
+    dmalloc((void**)&Lenv[i], sz * sizeof(T));
+    dmalloc((void**)&Renv[i], sz * sizeof(T));
+  }
+  Wd.assign(L, nullptr);
+  Wrd.assign(L, nullptr);
+  const size_t wsz = (size_t)D * d * d * D;
+  for (int i = 0; i < L; ++i) {
+    dmalloc((void**)&Wd[i], wsz * sizeof(T));
+    dmalloc((void**)&Wrd[i], wsz * sizeof(T));
+  }
+  if (variant == 2) {
+    const size_t K = (size_t)D * d * d;
+    WWd.assign(L - 1, nullptr);
+    for (int i = 0; i < L - 1; ++i) dmalloc((void**)&WWd[i], K * K * sizeof(T));
+    if (opt.precision == 1) {
+      WWf.assign(L - 1, nullptr);
+      for (int i = 0; i < L - 1; ++i) dmalloc((void**)&WWf[i], K * K * sizeof(float));
+      dmalloc((void**)&Lf, env_sz * sizeof(float));
+      dmalloc((void**)&Rf, env_sz * sizeof(float));
+      dmalloc((void**)&thf, nmax * sizeof(float));
+      dmalloc((void**)&outf, nmax * sizeof(float));
+      sc_f.cap = (size_t)D * d * d * chi * chi;
+      dmalloc((void**)&sc_f.X, sc_f.cap * sizeof(float));
+      dmalloc((void**)&sc_f.Y, sc_f.cap * sizeof(float));
+    }
+  }
+  dmalloc((void**)&theta, nmax * sizeof(T));
+  dmalloc((void**)&work, nmax * sizeof(T));
+  dmalloc((void**)&V, (size_t)kmax * nmax * sizeof(T));
+  sc_main.cap = (size_t)D * dv * chi * chi;
+  dmalloc((void**)&sc_main.X, sc_main.cap * sizeof(T));
+  dmalloc((void**)&sc_main.Y, sc_main.cap * sizeof(T));
+  sc_env.cap = (size_t)D * d * chi * chi;
+  dmalloc((void**)&sc_env.X, sc_env.cap * sizeof(T));
+  dmalloc((void**)&sc_env.Y, sc_env.cap * sizeof(T));
+  const size_t nb = (size_t)lz_blocks(nmax);
+  lz.alloc(std::max(kmax, 1), (size_t)(kmax + 1) * 2 * nb * sizeof(double), sizeof(T));
+  svdw.alloc(chi * d);
+  DMRG_HIP(hipHostMalloc((void**)&h_status, 4 * sizeof(int), hipHostMallocDefault));
+  DMRG_HIP(hipHostMalloc((void**)&h_scal, 8 * sizeof(double), hipHostMallocDefault));
+  runner.lz = &lz;
+  runner.V = V;
+  runner.work = work;
+  runner.h_status = h_status;
+  runner.h_scal = h_scal;
+  runner.beta_tol = opt.lanczos_tol;
+  runner.eig_tol = opt.eig_tol;
+}
+
+template <class T>
+void Engine<T>::free_all() {
+  if (s_main) (void)hipStreamSynchronize(s_main);
+  if (s_env) (void)hipStreamSynchronize(s_env);
+  auto f = [](void* p) { if (p) (void)hipFree(p); };
+  for (auto p : mps) f(p);
+  f(mps_spare);
+  for (auto p : Lenv) f(p);
+  for (auto p : Renv) f(p);
+  for (auto p : Wd) f(p);
+  for (auto p : Wrd) f(p);
+  for (auto p : WWd) f(p);
+  for (auto p : WWf) f(p);
+  f(Lf); f(Rf); f(thf); f(outf);
+  f(sc_f.X); f(sc_f.Y);
+  f(theta); f(work); f(V);
+  f(sc_main.X); f(sc_main.Y); f(sc_env.X); f(sc_env.Y);
+  mps.clear(); Lenv.clear(); Renv.clear(); Wd.clear(); Wrd.clear(); WWd.clear(); WWf.clear();
+  mps_spare = nullptr; Lf = Rf = thf = outf = nullptr; theta = work = V = nullptr;
+  sc_f.X = sc_f.Y = nullptr; sc_main.X = sc_main.Y = sc_env.X = sc_env.Y = nullptr;
+  lz.release();
+  svdw.release();
+  for (auto e : runner.ev) (void)hipEventDestroy(e);
+  runner.ev.clear();
+  if (h_status) (void)hipHostFree(h_status);
+  if (h_scal) (void)hipHostFree(h_scal);
+  h_status = nullptr;
+  h_scal = nullptr;
+  if (ev_canon) (void)hipEventDestroy(ev_canon);
+  if (ev_env) (void)hipEventDestroy(ev_env);
+  if (s_main) (void)hipStreamDestroy(s_main);
+  if (s_env) (void)hipStreamDestroy(s_env);
+  ev_canon = ev_env = nullptr;
+  s_main = s_env = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// MPO / MPS setup (dmrg2_gpu_impl.h:382-576)
+// ---------------------------------------------------------------------------
+template <class T>
+void Engine<T>::set_mpo(const T* const* W) {
+  DMRG_REQUIRE(W != nullptr, "set_mpo: null");
+  const size_t wsz = (size_t)D * d * d * D;
+  std::vector<T> wr(wsz);
+  for (int i = 0; i < L; ++i) {
+    DMRG_REQUIRE(W[i] != nullptr, "set_mpo: null site tensor");
+    for (size_t k = 0; k < wsz; ++k)
+      if (!finite_v(W[i][k])) throw Error(ST_NONFINITE, "set_mpo: non-finite MPO entry");
+    DMRG_HIP(hipMemcpy(Wd[i], W[i], wsz * sizeof(T), hipMemcpyHostToDevice));
+    build_wright<T>(W[i], d, D, wr.data());
+    DMRG_HIP(hipMemcpy(Wrd[i], wr.data(), wsz * sizeof(T), hipMemcpyHostToDevice));
+  }
+  if (variant == 2) {
+    const size_t K = (size_t)D * d * d;
+    std::vector<T> ww(K * K);
+    std::vector<float> wwf(K * K);
+    for (int i = 0; i < L - 1; ++i) {
+      build_ww2<T>(W[i], W[i + 1], d, D, ww.data());
+      DMRG_HIP(hipMemcpy(WWd[i], ww.data(), K * K * sizeof(T), hipMemcpyHostToDevice));
+      if (opt.precision == 1) {
+        for (size_t k = 0; k < K * K; ++k) wwf[k] = (float)real(ww[k]);
+        DMRG_HIP(hipMemcpy(WWf[i], wwf.data(), K * K * sizeof(float), hipMemcpyHostToDevice));
+      }
+    }
+  }
+  mpo_set = true;
+}
+
+template <class T>
+void Engine<T>::init_random(uint64_t seed, double scale) {
+  uint64_t off = 0;
+  for (int i = 0; i < L; ++i) {
+    size_t n = (size_t)bond[i] * d * bond[i + 1];
+    fill_random<T>(s_main, mps[i], n, seed, off, scale);
+    off += n;
+  }
+  DMRG_HIP(hipStreamSynchronize(s_main));
+}
+
+template <class T>
+void Engine<T>::init_product() {
+  for (int i = 1; i < L; ++i) bond[i] = 1;
+  std::vector<T> h(d, T(0));
+  h[0] = T(1);
+  for (int i = 0; i < L; ++i) DMRG_HIP(hipMemcpy(mps[i], h.data(), d * sizeof(T), hipMemcpyHostToDevice));
+}
+
+template <class T>
+void Engine<T>::init_neel() {
+  for (int i = 0; i < L; ++i) {
+    int cL = bond[i], cR = bond[i + 1];
+    std::vector<T> h((size_t)cL * d * cR, T(0));
+    int s = i % 2;
+    for (int a = 0; a < std::min(cL, cR); ++a) h[a + (size_t)s * cL + (size_t)a * cL * d] = T(1);
+    DMRG_HIP(hipMemcpy(mps[i], h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+}
+
+template <class T>
+void Engine<T>::set_mps(int site, const T* A, int cL, int cR) {
+  DMRG_REQUIRE(site >= 0 && site < L, "set_mps: bad site");
+  DMRG_REQUIRE(cL >= 1 && cR >= 1 && cL <= chi && cR <= chi, "set_mps: bond dims out of range");
+  DMRG_REQUIRE(site != 0 || cL == 1, "set_mps: first site needs cL = 1");
+  DMRG_REQUIRE(site != L - 1 || cR == 1, "set_mps: last site needs cR = 1");
+  size_t n = (size_t)cL * d * cR;
+  for (size_t k = 0; k < n; ++k)
+    if (!finite_v(A[k])) throw Error(ST_NONFINITE, "set_mps: non-finite entry");
+  bond[site] = cL;
+  bond[site + 1] = cR;
+  DMRG_HIP(hipMemcpy(mps[site], A, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+template <class T>
+void Engine<T>::get_mps(int site, T* out) const {
+  DMRG_REQUIRE(site >= 0 && site < L, "get_mps: bad site");
+  size_t n = (size_t)bond[site] * d * bond[site + 1];
+  DMRG_HIP(hipStreamSynchronize(s_main));
+  DMRG_HIP(hipStreamSynchronize(s_env));
+  DMRG_HIP(hipMemcpy(out, mps[site], n * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+// ---------------------------------------------------------------------------
+// Environments (dmrg2_gpu_impl.h:786-964)
+// ---------------------------------------------------------------------------
+template <class T>
+void Engine<T>::env_left_at(int site) {
+  env_left<T>(s_env, Lenv[site], mps[site], Wd[site], Lenv[site + 1], bond[site], bond[site + 1], d,
+              D, sc_env);
+  stats.env_flop += flops_env(bond[site], bond[site + 1], d, D, is_cplx<T>::value);
+}
+
+template <class T>
+void Engine<T>::env_right_at(int site) {
+  env_right<T>(s_env, Renv[site + 1], mps[site], Wrd[site], Renv[site], bond[site + 1], bond[site], d,
+               D, sc_env);
+  stats.env_flop += flops_env(bond[site + 1], bond[site], d, D, is_cplx<T>::value);
+}
+
+template <class T>
+void Engine<T>::build_envs() {
+  set_boundary_env<T>(s_env, Lenv[0], D, 0);
+  set_boundary_env<T>(s_env, Renv[L], D, D - 1);
+  for (int i = L - 1; i >= 0; --i) env_right_at(i);
+  DMRG_HIP(hipStreamSynchronize(s_env));
+  env_pending = false;
+}
+
+template <class T>
+void Engine<T>::wait_env() {
+  if (env_pending) {
+    DMRG_HIP(hipStreamWaitEvent(s_main, ev_env, 0));
+    env_pending = false;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-site optimisation (dmrg2_gpu_impl.h:582-1359)
+// ---------------------------------------------------------------------------
+template <class T>
+void Engine<T>::matvec2(int site, const T* in, T* out, bool fp64) {
+  const int cL = bond[site], cR = bond[site + 2];
+  if (fp64 || opt.precision == 0) {
+    heff2_apply<T>(s_main, Lenv[site], Renv[site + 2], WWd[site], in, out, cL, cR, d, D, sc_main);
+  } else {
+    if constexpr (!is_cplx<T>::value) {
+      const size_t n = (size_t)cL * d * d * cR;
+      convert_f64_f32(s_main, in, thf, n);
+      heff2_apply<float>(s_main, Lf, Rf, WWf[site], thf, outf, cL, cR, d, D, sc_f);
+      convert_f32_f64(s_main, outf, out, n);
+    }
+  }
+}
+
+template <class T>
+double Engine<T>::lanczos_two(int site, bool rq) {
+  const int cL = bond[site], cR = bond[site + 2];
+  const size_t n = (size_t)cL * d * d * cR;
+  if (opt.precision == 1) {
+    if constexpr (!is_cplx<T>::value) {
+      convert_f64_f32(s_main, Lenv[site], Lf, (size_t)cL * D * cL);
+      convert_f64_f32(s_main, Renv[site + 2], Rf, (size_t)cR * D * cR);
+    }
+  }
+  double t0 = now_s();
+  double E = runner.run(s_main, theta, n, opt.krylov_max, site,
+                        [&](const T* in, T* out) { matvec2(site, in, out, false); });
+  stats.lanczos_ms += 1e3 * (now_s() - t0);
+  stats.heff_ms += runner.last_heff_ms;
+  stats.matvecs += runner.last_matvecs;
+  stats.lanczos_calls += 1;
+  stats.heff_flop += runner.last_matvecs * flops_heff2(cL, cR, d, D, is_cplx<T>::value);
+  if (rq && opt.precision != 0) E = rayleigh2(site);
+  return E;
+}
+
+// FP64 Rayleigh quotient <theta|H|theta> of the (normalised) Ritz vector.
+template <class T>
+double Engine<T>::rayleigh2(int site) {
+  const int cL = bond[site], cR = bond[site + 2];
+  const size_t n = (size_t)cL * d * d * cR;
+  matvec2(site, theta, work, true);
+  lz_alpha<T>(s_main, lz, theta, work, n, 0);
+  double a = 0.0;
+  DMRG_HIP(hipMemcpyAsync(&a, lz.alpha, sizeof(double), hipMemcpyDeviceToHost, s_main));
+  DMRG_HIP(hipStreamSynchronize(s_main));
+  return a;
+}
+
+template <class T>
+void Engine<T>::split_two(int site, char dir) {
+  const int cL = bond[site], cR = bond[site + 2];
+  const int m = cL * d, n = d * cR;
+  const int full_k = std::min(m, n), k = std::min(full_k, chi);
+  double t0 = now_s();
+  SvdResult r = svd_jacobi<T>(s_main, svdw, theta, m, n, m, opt.svd_max_sweeps);
+  int nk = svd_rank<T>(s_main, svdw, r, k, opt.svd_cutoff, opt.svd_rel_cutoff);
+  svd_write<T>(s_main, svdw, r, m, n, nk, dir == 'L', mps[site], m, mps[site + 1], nk);
+  bond[site + 1] = nk;
+  DMRG_HIP(hipEventRecord(ev_canon, s_main));
+  stats.svd_ms += 1e3 * (now_s() - t0);
+  stats.svd_calls += 1;
+  stats.svd_sweeps += r.sweeps;
+}
+
+template <class T>
+double Engine<T>::optimize_bond(int site, char dir, bool rq) {
+  wait_env();
+  const int cL = bond[site], cm = bond[site + 1], cR = bond[site + 2];
+  gemm<T>(s_main, OP_N, OP_N, cL * d, d * cR, cm, T(1), mps[site], cL * d, mps[site + 1], cm, T(0),
+          theta, cL * d, gemm_pick_tile(cL * d, d * cR, cm, is_cplx<T>::value));
+  double E = lanczos_two(site, rq);
+  split_two(site, dir);
+  return E;
+}
+
+template <class T>
+double Engine<T>::sweep2_lr() {
+  double E = 0.0;
+  for (int site = 0; site < L - 1; ++site) {
+    E = optimize_bond(site, 'R', false);
+    DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));
+    env_left_at(site);
+    DMRG_HIP(hipEventRecord(ev_env, s_env));
+    env_pending = true;
+  }
+  wait_env();
+  return E;
+}
+
+template <class T>
+double Engine<T>::sweep2_rl() {
+  double E = 0.0;
+  for (int site = L - 2; site >= 0; --site) {
+    E = optimize_bond(site, 'L', site == 0);
+    DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));
+    env_right_at(site + 1);
+    DMRG_HIP(hipEventRecord(ev_env, s_env));
+    env_pending = true;
+  }
+  wait_env();
+  return E;
+}
+
+// ---------------------------------------------------------------------------
+// Single-site optimisation (dmrg_gpu_impl.h:895-1445)
+// ---------------------------------------------------------------------------
+template <class T>
+double Engine<T>::lanczos_one(int site) {
+  const int cL = bond[site], cR = bond[site + 1];
+  const size_t n = (size_t)cL * d * cR;
+  double t0 = now_s();
+  double E = runner.run(s_main, theta, n, opt.krylov_max, site, [&](const T* in, T* out) {
+    heff1_apply<T>(s_main, Lenv[site], Renv[site + 1], Wd[site], in, out, cL, cR, d, D, sc_main);
+  });
+  stats.lanczos_ms += 1e3 * (now_s() - t0);
+  stats.heff_ms += runner.last_heff_ms;
+  stats.matvecs += runner.last_matvecs;
+  stats.lanczos_calls += 1;
+  stats.heff_flop += runner.last_matvecs * flops_heff1(cL, cR, d, D, is_cplx<T>::value);
+  return E;
+}
+
+template <class T>
+void Engine<T>::split_one(int site, char dir) {
+  const int cL = bond[site], cR = bond[site + 1];
+  double t0 = now_s();
+  if (dir == 'R') {
+    const int m = cL * d, n = cR;
+    const int k = std::min(std::min(m, n), chi);
+    SvdResult r = svd_jacobi<T>(s_main, svdw, theta, m, n, m, opt.svd_max_sweeps);
+    int nk = svd_rank<T>(s_main, svdw, r, k, opt.svd_cutoff, opt.svd_rel_cutoff);
+    // A[site] = U; S*Vh (nk x cR) -> work; A[site+1] <- (S Vh) A[site+1]
+    svd_write<T>(s_main, svdw, r, m, n, nk, false, mps[site], m, work, nk);
+    DMRG_HIP(hipEventRecord(ev_canon, s_main));
+    const int cR2 = bond[site + 2];
+    gemm<T>(s_main, OP_N, OP_N, nk, d * cR2, cR, T(1), work, nk, mps[site + 1], cR, T(0), mps_spare,
+            nk, gemm_pick_tile(nk, d * cR2, cR, is_cplx<T>::value));
+    std::swap(mps[site + 1], mps_spare);
+    bond[site + 1] = nk;
+    stats.svd_sweeps += r.sweeps;
+  } else {
+    const int m = cL, n = d * cR;
+    const int k = std::min(std::min(m, n), chi);
+    SvdResult r = svd_jacobi<T>(s_main, svdw, theta, m, n, m, opt.svd_max_sweeps);
+    int nk = svd_rank<T>(s_main, svdw, r, k, opt.svd_cutoff, opt.svd_rel_cutoff);
+    // A[site] = Vh; U*S (cL x nk) -> work; A[site-1] <- A[site-1] (U S)
+    svd_write<T>(s_main, svdw, r, m, n, nk, true, work, m, mps[site], nk);
+    DMRG_HIP(hipEventRecord(ev_canon, s_main));
+    const int cL0 = bond[site - 1];
+    gemm<T>(s_main, OP_N, OP_N, cL0 * d, nk, cL, T(1), mps[site - 1], cL0 * d, work, cL, T(0),
+            mps_spare, cL0 * d, gemm_pick_tile(cL0 * d, nk, cL, is_cplx<T>::value));
+    std::swap(mps[site - 1], mps_spare);
+    bond[site] = nk;
+    stats.svd_sweeps += r.sweeps;
+  }
+  stats.svd_ms += 1e3 * (now_s() - t0);
+  stats.svd_calls += 1;
+}
+
+template <class T>
+double Engine<T>::optimize_site(int site, char dir, bool split) {
+  wait_env();
+  const size_t n = (size_t)bond[site] * d * bond[site + 1];
+  DMRG_HIP(hipMemcpyAsync(theta, mps[site], n * sizeof(T), hipMemcpyDeviceToDevice, s_main));
+  double E = lanczos_one(site);
+  if (split) {
+    split_one(site, dir);
+  } else {
+    DMRG_HIP(hipMemcpyAsync(mps[site], theta, n * sizeof(T), hipMemcpyDeviceToDevice, s_main));
+  }
+  return E;
+}
+
+template <class T>
+double Engine<T>::sweep1_lr() {
+  for (int site = 0; site < L - 1; ++site) {
+    optimize_site(site, 'R', true);
+    DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));
+    env_left_at(site);
+    DMRG_HIP(hipEventRecord(ev_env, s_env));
+    env_pending = true;
+  }
+  return optimize_site(L - 1, 'R', false);
+}
+
+template <class T>
+double Engine<T>::sweep1_rl() {
+  for (int site = L - 1; site >= 1; --site) {
+    optimize_site(site, 'L', true);
+    DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));
+    env_right_at(site);
+    DMRG_HIP(hipEventRecord(ev_env, s_env));
+    env_pending = true;
+  }
+  return optimize_site(0, 'L', false);
+}
+
+// ---------------------------------------------------------------------------
+// run (dmrg2_gpu_impl.h:1426-1457)
+// ---------------------------------------------------------------------------
+template <class T>
+double Engine<T>::run(int n_sweeps, bool quiet) {
+  if (!mpo_set) throw Error(ST_STATE, "run: set_mpo has not been called");
+  DMRG_REQUIRE(n_sweeps >= 0, "run: negative sweep count");
+  double te = now_s();
+  build_envs();
+  double t_start = now_s();
+  double env_time = t_start - te;
+  stats.env_build_s = env_time;
+  if (!quiet) printf("  Environment build: %.3f s\n", env_time);
+  double e_prev = 0.0;
+  for (int sweep = 0; sweep < n_sweeps; ++sweep) {
+    double ts = now_s();
+    if (variant == 2) {
+      sweep2_lr();
+      energy = sweep2_rl();
+    } else {
+      sweep1_lr();
+      energy = sweep1_rl();
+    }
+    DMRG_HIP(hipStreamSynchronize(s_main));
+    DMRG_HIP(hipStreamSynchronize(s_env));
+    stats.sweep_s_last = now_s() - ts;
+    stats.sweeps_s_total += stats.sweep_s_last;
+    stats.sweeps_done += 1;
+    sweep_E.push_back(energy);
+    if (!std::isfinite(energy)) throw Error(ST_NONFINITE, "run: non-finite sweep energy");
+    double dE = std::fabs(energy - e_prev);
+    if (dE < tol && sweep > 0) break;
+    e_prev = energy;
+  }
+  double total = now_s() - t_start;
+  if (!quiet) {
+    printf("Final energy: %.12f\n", energy);
+    printf("Total wall time: %.3f s\n", total);
+    printf("  env_build_sec: %.3f  timer_scope: sweep_only\n", env_time);
+    fflush(stdout);
+  }
+  return energy;
+}
+
+// ---------------------------------------------------------------------------
+// <psi|H|psi> / <psi|psi> by full left contraction (FP64).
+// ---------------------------------------------------------------------------
+template <class T>
+void Engine<T>::expectation(double* e, double* nrm2) {
+  if (!mpo_set) throw Error(ST_STATE, "expectation: set_mpo has not been called");
+  DMRG_HIP(hipStreamSynchronize(s_main));
+  // Use Lenv[1..] as scratch is not allowed (they hold state); ping-pong in
+  // the spare MPS-sized buffers instead: env (chi, D, chi) fits in work/theta
+  // only when D <= d*d (two-site) -- use dedicated temporaries.
+  const size_t esz = (size_t)chi * D * chi;
+  T *ea = nullptr, *eb = nullptr, *wid = nullptr;
+  DMRG_HIP(hipMalloc(&ea, esz * sizeof(T)));
+  DMRG_HIP(hipMalloc(&eb, esz * sizeof(T)));
+  DMRG_HIP(hipMalloc(&wid, (size_t)d * d * sizeof(T)));
+  std::vector<T> idm((size_t)d * d, T(0));
+  for (int s = 0; s < d; ++s) idm[s + (size_t)d * s] = T(1);
+  DMRG_HIP(hipMemcpy(wid, idm.data(), idm.size() * sizeof(T), hipMemcpyHostToDevice));
+  T hv[2];
+  for (int pass = 0; pass < 2; ++pass) {
+    const int Dp = pass == 0 ? D : 1;
+    set_boundary_env<T>(s_main, ea, Dp, 0);
+    for (int i = 0; i < L; ++i) {
+      env_left<T>(s_main, ea, mps[i], pass == 0 ? Wd[i] : wid, eb, bond[i], bond[i + 1], d, Dp,
+                  sc_main);
+      std::swap(ea, eb);
+    }
+    DMRG_HIP(hipMemcpyAsync(&hv[pass], ea + (Dp - 1), sizeof(T), hipMemcpyDeviceToHost, s_main));
+    DMRG_HIP(hipStreamSynchronize(s_main));
+  }
+  (void)hipFree(ea);
+  (void)hipFree(eb);
+  (void)hipFree(wid);
+  *nrm2 = real(hv[1]);
+  *e = real(hv[0]) / real(hv[1]);
+}
+
+template <class T>
+int Engine<T>::get_env(bool left, int idx, T* out) {
+  DMRG_REQUIRE(idx >= 0 && idx <= L, "get_env: bad index");
+  const int c = bond[idx];
+  if (out) {
+    DMRG_HIP(hipStreamSynchronize(s_main));
+    DMRG_HIP(hipStreamSynchronize(s_env));
+    DMRG_HIP(hipMemcpy(out, left ? Lenv[idx] : Renv[idx], (size_t)c * D * c * sizeof(T),
+                       hipMemcpyDeviceToHost));
+  }
+  return c;
+}
+
+template class Engine<double>;
+template class Engine<cdouble>;
+template struct LanczosRunner<double>;
+template struct LanczosRunner<cdouble>;
+
+}  // namespace dmrg

@@ -1,0 +1,138 @@
+// DMRG sweep engine (host side, C++), the MI355X counterpart of the
+// reference's DMRG2GPU<Scalar> (gpu-rocm/dmrg2-gpu/src/dmrg2_gpu.h:23-201) and
+// DMRGGPU<Scalar> (gpu-rocm/dmrg-gpu/src/dmrg_gpu.h:23-219).
+#pragma once
+#include <chrono>
+#include <functional>
+#include <vector>
+
+#include "common.hpp"
+#include "contract.hpp"
+#include "lanczos.hpp"
+#include "svd.hpp"
+
+namespace dmrg {
+
+struct Options {
+  int krylov_max = 100;
+  double lanczos_tol = 1e-12;
+  double eig_tol = 1e-12;
+  double svd_cutoff = 1e-14;
+  double svd_rel_cutoff = 0.0;
+  int precision = 0;  // 0 fp64, 1 fp32 matvec
+  int profile = 0;
+  int device = 0;
+  int svd_max_sweeps = 40;
+};
+
+struct Stats {
+  double lanczos_ms = 0, heff_ms = 0, svd_ms = 0, env_ms = 0, sweep_s_last = 0;
+  double heff_flop = 0, env_flop = 0;
+  long long matvecs = 0, lanczos_calls = 0, svd_calls = 0, svd_sweeps = 0;
+  int sweeps_done = 0;
+  double max_discarded = 0;
+  double env_build_s = 0, sweeps_s_total = 0;
+};
+
+// Algorithmic flop counts (real arithmetic; complex = 4x).
+inline double flops_heff2(int cL, int cR, int d, int D, bool cplx) {
+  double dd = (double)d * d;
+  double f = 2.0 * D * dd * cL * cR * ((double)cL + cR) + 2.0 * cL * cR * (D * dd) * (D * dd);
+  return cplx ? 4 * f : f;
+}
+inline double flops_heff1(int cL, int cR, int d, int D, bool cplx) {
+  double f = 2.0 * D * d * cL * (double)cR * ((double)cL + cR) + 2.0 * cL * (double)cR * (D * d) * (D * d);
+  return cplx ? 4 * f : f;
+}
+inline double flops_env(int ci, int co, int d, int D, bool cplx) {
+  double f = 2.0 * D * d * ci * (double)co * ((double)ci + co) + 2.0 * ci * (double)co * (D * d) * (D * d);
+  return cplx ? 4 * f : f;
+}
+
+// One Lanczos solve (reference semantics, dmrg2_gpu_impl.h:970-1160) with a
+// caller-supplied matvec.  theta (length n) is replaced by the Ritz vector.
+template <class T>
+struct LanczosRunner {
+  LzDev* lz = nullptr;
+  T* V = nullptr;      // kmax * n
+  T* work = nullptr;   // n
+  int* h_status = nullptr;   // pinned [4]
+  double* h_scal = nullptr;  // pinned [8]
+  std::vector<hipEvent_t> ev;  // 2 per matvec
+  double beta_tol = 1e-12, eig_tol = 1e-12;
+  int last_niter = 0, last_matvecs = 0;
+  double last_heff_ms = 0;
+  double run(hipStream_t st, T* theta, size_t n, int kmax, int site,
+             const std::function<void(const T*, T*)>& matvec);
+};
+
+template <class T>
+class Engine {
+ public:
+  Engine(int L, int d, int chi, int D, double tol, int variant, const Options& opt);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  void set_mpo(const T* const* W);
+  void init_random(uint64_t seed, double scale);
+  void init_product();
+  void init_neel();
+  void set_mps(int site, const T* A, int cL, int cR);
+  double run(int n_sweeps, bool quiet);
+  void get_mps(int site, T* out) const;
+  void expectation(double* e, double* nrm2);
+  int get_env(bool left, int idx, T* out);
+
+  int L, d, chi, D;
+  double tol;
+  int variant;
+  Options opt;
+  std::vector<int> bond;
+  double energy = 0.0;
+  std::vector<double> sweep_E;
+  Stats stats;
+
+ private:
+  void alloc_all();
+  void free_all();
+  void build_envs();
+  void wait_env();
+  void env_left_at(int site);
+  void env_right_at(int site);  // builds R[site] from R[site+1] and A[site]
+  double optimize_bond(int site, char dir, bool rq);
+  double optimize_site(int site, char dir, bool split);
+  double sweep2_lr();
+  double sweep2_rl();
+  double sweep1_lr();
+  double sweep1_rl();
+  double lanczos_two(int site, bool rq);
+  double lanczos_one(int site);
+  void matvec2(int site, const T* in, T* out, bool fp64);
+  void split_two(int site, char dir);
+  void split_one(int site, char dir);
+  double rayleigh2(int site);
+
+  hipStream_t s_main = nullptr, s_env = nullptr;
+  hipEvent_t ev_canon = nullptr, ev_env = nullptr;
+  bool env_pending = false;
+  bool mpo_set = false;
+
+  std::vector<T*> mps;
+  T* mps_spare = nullptr;
+  std::vector<T*> Lenv, Renv, Wd, Wrd, WWd;
+  std::vector<float*> WWf;
+  float *Lf = nullptr, *Rf = nullptr, *thf = nullptr, *outf = nullptr;
+  T *theta = nullptr, *work = nullptr, *V = nullptr;
+  ContractScratch<T> sc_main, sc_env;
+  ContractScratch<float> sc_f;
+  LzDev lz;
+  SvdWork<T> svdw;
+  LanczosRunner<T> runner;
+  int* h_status = nullptr;
+  double* h_scal = nullptr;
+  size_t nmax = 0;
+  int kmax = 0;
+};
+
+}  // namespace dmrg

@@ -1,0 +1,347 @@
+// MFMA GEMM kernels for gfx950 (see gemm.hpp).
+//
+// Workgroup: 256 threads = 4 wave64s in a 2x2 grid.  Each wave owns a
+// (BM/2) x (BN/2) sub-tile made of 16x16 MFMA blocks.  Operand tiles of depth
+// BK=16 are staged through LDS (double buffered, one-element row padding),
+// with the global loads of the next tile issued before the MFMAs of the
+// current one.  The MFMA is issued with the B fragment in the "A" slot so
+// that each lane's four accumulator registers land on four columns of C with
+// consecutive lanes on consecutive rows: epilogue stores of a column-major C
+// are 128-byte runs.
+#include "gemm.hpp"
+
+namespace dmrg {
+namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <class T> struct GT;
+// G1: accumulator register r of lane l holds output row (l/16) + 4r of the
+// 16x16 block (f64 MFMA, group size 1); otherwise row 4(l/16) + r (f32 MFMA,
+// group size 4).  Column = l % 16 in both cases.
+template <> struct GT<double> { using E = double; using V4 = d4_t; static constexpr int P = 1; static constexpr bool G1 = true; };
+template <> struct GT<cdouble> { using E = double; using V4 = d4_t; static constexpr int P = 2; static constexpr bool G1 = true; };
+template <> struct GT<float> { using E = float; using V4 = f4_t; static constexpr int P = 1; static constexpr bool G1 = false; };
+
+__device__ inline d4_t mfma(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline f4_t mfma(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class T> __device__ inline void split(T v, typename GT<T>::E* out);
+template <> __device__ inline void split<double>(double v, double* o) { o[0] = v; }
+template <> __device__ inline void split<float>(float v, float* o) { o[0] = v; }
+template <> __device__ inline void split<cdouble>(cdouble v, double* o) { o[0] = v.x; o[1] = v.y; }
+
+template <class T> __device__ inline T conj_v(T v) { return conj(v); }
+template <> __device__ inline float conj_v<float>(float v) { return v; }
+
+// Plain column-major operand; TR: stored transposed.
+template <bool TR> struct AddrPlain {
+  long long ld;
+  __device__ long long operator()(int r, int c) const {
+    return TR ? (long long)c + (long long)r * ld : (long long)r + (long long)c * ld;
+  }
+};
+// Two-level operand.
+struct AddrTensor {
+  View2 v;
+  __device__ static long long dim(const Dim2& d, int i) {
+    return (long long)(i % d.n1) * d.s1 + (long long)(i / d.n1) * d.s2;
+  }
+  __device__ long long operator()(int r, int c) const { return dim(v.row, r) + dim(v.col, c); }
+};
+
+constexpr int BK = 16;
+
+template <class T, bool CA, bool CB, bool KFA, bool KFB, int BM, int BN, class AA, class AB, class AC>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A,
+                                              AA aa, long long sA, const T* __restrict__ B, AB ab,
+                                              long long sB, T beta, T* __restrict__ C, AC ac,
+                                              long long sC) {
+  using E = typename GT<T>::E;
+  using V4 = typename GT<T>::V4;
+  constexpr int P = GT<T>::P;
+  constexpr int PADM = BM + 1, PADN = BN + 1;
+  constexpr int FM = BM / 32, FN = BN / 32;  // 16x16 blocks per wave
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+
+  __shared__ E As[2][P][BK][PADM];
+  __shared__ E Bs[2][P][BK][PADN];
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  A += blockIdx.z * sA;
+  B += blockIdx.z * sB;
+  C += blockIdx.z * sC;
+
+  T ra[NA], rb[NB];
+
+  auto idxA = [&](int r, int& m, int& k) {
+    if (!KFA) { m = t % BM; k = t / BM + r * (256 / BM); }
+    else      { k = t % BK; m = t / BK + r * (256 / BK); }
+  };
+  auto idxB = [&](int r, int& n, int& k) {
+    if (!KFB) { n = t % BN; k = t / BN + r * (256 / BN); }
+    else      { k = t % BK; n = t / BK + r * (256 / BK); }
+  };
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      int m, k;
+      idxA(r, m, k);
+      int gm = m0 + m, gk = k0 + k;
+      T v = T(0);
+      if (gm < M && gk < K) v = A[aa(gm, gk)];
+      ra[r] = CA ? conj_v(v) : v;
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      int n, k;
+      idxB(r, n, k);
+      int gn = n0 + n, gk = k0 + k;
+      T v = T(0);
+      if (gn < N && gk < K) v = B[ab(gk, gn)];
+      rb[r] = CB ? conj_v(v) : v;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      int m, k;
+      idxA(r, m, k);
+      E e[P];
+      split<T>(ra[r], e);
+#pragma unroll
+      for (int p = 0; p < P; ++p) As[buf][p][k][m] = e[p];
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      int n, k;
+      idxB(r, n, k);
+      E e[P];
+      split<T>(rb[r], e);
+#pragma unroll
+      for (int p = 0; p < P; ++p) Bs[buf][p][k][n] = e[p];
+    }
+  };
+
+  V4 acc[P][FM][FN];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[p][i][j] = V4{0, 0, 0, 0};
+
+  const int nk = (K + BK - 1) / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int li = lane & 15, lk = lane >> 4;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      E af[P][FM], bf[P][FN];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[p][i] = As[buf][p][kk + lk][wm * (BM / 2) + i * 16 + li];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[p][j] = Bs[buf][p][kk + lk][wn * (BN / 2) + j * 16 + li];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if constexpr (P == 1) {
+            acc[0][i][j] = mfma(bf[0][j], af[0][i], acc[0][i][j]);
+          } else {
+            acc[0][i][j] = mfma(bf[0][j], af[0][i], acc[0][i][j]);
+            acc[0][i][j] = mfma(-bf[1][j], af[1][i], acc[0][i][j]);
+            acc[1][i][j] = mfma(bf[1][j], af[0][i], acc[1][i][j]);
+            acc[1][i][j] = mfma(bf[0][j], af[1][i], acc[1][i][j]);
+          }
+        }
+    }
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue: lane -> row li of C; register r -> column lk + 4r (f64) or
+  // 4*lk + r (f32) of each 16x16 block.
+  const bool use_beta = !(real(beta) == 0.0 && imag(beta) == 0.0);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int gm = m0 + wm * (BM / 2) + i * 16 + li;
+        int gn = n0 + wn * (BN / 2) + j * 16 + (GT<T>::G1 ? lk + 4 * r : 4 * lk + r);
+        if (gm < M && gn < N) {
+          T v;
+          if constexpr (P == 1) v = T(acc[0][i][j][r]);
+          else v = make_scalar<T>(acc[0][i][j][r], acc[1][i][j][r]);
+          v = alpha * v;
+          long long idx = ac(gm, gn);
+          if (use_beta) v = v + beta * C[idx];
+          C[idx] = v;
+        }
+      }
+}
+
+template <class T, bool CA, bool CB, bool KFA, bool KFB, class AA, class AB, class AC>
+void launch(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* A, AA aa,
+            long long sA, const T* B, AB ab, long long sB, T beta, T* C, AC ac, long long sC,
+            int batch) {
+#define DMRG_GEMM_LAUNCH(BM_, BN_)                                                          \
+  hipLaunchKernelGGL((k_gemm<T, CA, CB, KFA, KFB, BM_, BN_, AA, AB, AC>),                   \
+                     dim3(cdiv(M, BM_), cdiv(N, BN_), batch), dim3(256), 0, st, M, N, K,    \
+                     alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC)
+  if (tl.bm == 32 && tl.bn == 32) {
+    DMRG_GEMM_LAUNCH(32, 32);
+  } else if (tl.bm == 128 && tl.bn == 64 && !is_cplx<T>::value) {
+    DMRG_GEMM_LAUNCH(128, 64);
+  } else {
+    DMRG_GEMM_LAUNCH(64, 64);
+  }
+#undef DMRG_GEMM_LAUNCH
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T, bool TA, bool TB, bool CA, bool CB>
+void plain4(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* A, int lda,
+            long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc, long long sC,
+            int batch) {
+  launch<T, CA, CB, TA, !TB>(st, tl, M, N, K, alpha, A, AddrPlain<TA>{lda}, sA, B,
+                             AddrPlain<TB>{ldb}, sB, beta, C, AddrPlain<false>{ldc}, sC, batch);
+}
+
+template <class T>
+void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T alpha, const T* A,
+           int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc,
+           long long sC, int batch) {
+  DMRG_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "gemm: negative size");
+  DMRG_REQUIRE(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "gemm: bad op");
+  if (M == 0 || N == 0 || batch == 0) return;
+  const bool ta = op_trans(opA), tb = op_trans(opB);
+  bool ca = op_conj(opA), cb = op_conj(opB);
+  if (!is_cplx<T>::value) ca = cb = false;
+#define DMRG_P4(TA_, TB_, CA_, CB_) \
+  plain4<T, TA_, TB_, CA_, CB_>(st, tl, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch)
+  if (!ca && !cb) {
+    if (!ta && !tb) DMRG_P4(false, false, false, false);
+    else if (!ta && tb) DMRG_P4(false, true, false, false);
+    else if (ta && !tb) DMRG_P4(true, false, false, false);
+    else DMRG_P4(true, true, false, false);
+  } else {
+    if constexpr (is_cplx<T>::value) {
+      int code = (ta ? 8 : 0) | (tb ? 4 : 0) | (ca ? 2 : 0) | (cb ? 1 : 0);
+      switch (code) {
+        case 1: DMRG_P4(false, false, false, true); break;
+        case 2: DMRG_P4(false, false, true, false); break;
+        case 3: DMRG_P4(false, false, true, true); break;
+        case 5: DMRG_P4(false, true, false, true); break;
+        case 6: DMRG_P4(false, true, true, false); break;
+        case 7: DMRG_P4(false, true, true, true); break;
+        case 9: DMRG_P4(true, false, false, true); break;
+        case 10: DMRG_P4(true, false, true, false); break;
+        case 11: DMRG_P4(true, false, true, true); break;
+        case 13: DMRG_P4(true, true, false, true); break;
+        case 14: DMRG_P4(true, true, true, false); break;
+        default: DMRG_P4(true, true, true, true); break;
+      }
+    }
+  }
+#undef DMRG_P4
+}
+
+}  // namespace
+
+GemmTile gemm_pick_tile(int M, int N, int K, bool complex_) {
+  (void)K;
+  long long t64 = (long long)cdiv(M, 64) * cdiv(N, 64);
+  if (t64 < 256) return {32, 32};
+  if (!complex_) {
+    long long t128 = (long long)cdiv(M, 128) * cdiv(N, 64);
+    if (t128 >= 1024) return {128, 64};
+  }
+  return {64, 64};
+}
+
+template <class T>
+void gemm(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const T* A, int lda,
+          const T* B, int ldb, T beta, T* C, int ldc, GemmTile tile) {
+  plain<T>(st, opA, opB, tile, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1);
+}
+
+template <class T>
+void gemm_strided(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const T* A,
+                  int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc,
+                  long long sC, int batch, GemmTile tile) {
+  plain<T>(st, opA, opB, tile, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+template <class T>
+void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2 va, bool conjA,
+                 bool a_kfast, const T* B, View2 vb, bool conjB, bool b_kfast, T beta, T* C,
+                 View2 vc, GemmTile tile) {
+  DMRG_REQUIRE(M >= 0 && N >= 0 && K >= 0, "gemm_tensor: negative size");
+  if (M == 0 || N == 0) return;
+  if (!is_cplx<T>::value) conjA = conjB = false;
+  AddrTensor aa{va}, ab{vb}, ac{vc};
+#define DMRG_GT(CA_, CB_, KA_, KB_) \
+  launch<T, CA_, CB_, KA_, KB_>(st, tile, M, N, K, alpha, A, aa, 0, B, ab, 0, beta, C, ac, 0, 1)
+  int code = (conjA ? 8 : 0) | (conjB ? 4 : 0) | (a_kfast ? 2 : 0) | (b_kfast ? 1 : 0);
+  if constexpr (is_cplx<T>::value) {
+    switch (code) {
+      case 0: DMRG_GT(false, false, false, false); break;
+      case 1: DMRG_GT(false, false, false, true); break;
+      case 2: DMRG_GT(false, false, true, false); break;
+      case 3: DMRG_GT(false, false, true, true); break;
+      case 4: DMRG_GT(false, true, false, false); break;
+      case 5: DMRG_GT(false, true, false, true); break;
+      case 6: DMRG_GT(false, true, true, false); break;
+      case 7: DMRG_GT(false, true, true, true); break;
+      case 8: DMRG_GT(true, false, false, false); break;
+      case 9: DMRG_GT(true, false, false, true); break;
+      case 10: DMRG_GT(true, false, true, false); break;
+      case 11: DMRG_GT(true, false, true, true); break;
+      case 12: DMRG_GT(true, true, false, false); break;
+      case 13: DMRG_GT(true, true, false, true); break;
+      case 14: DMRG_GT(true, true, true, false); break;
+      default: DMRG_GT(true, true, true, true); break;
+    }
+  } else {
+    switch (code & 3) {
+      case 0: DMRG_GT(false, false, false, false); break;
+      case 1: DMRG_GT(false, false, false, true); break;
+      case 2: DMRG_GT(false, false, true, false); break;
+      default: DMRG_GT(false, false, true, true); break;
+    }
+  }
+#undef DMRG_GT
+}
+
+#define DMRG_INST(T)                                                                         \
+  template void gemm<T>(hipStream_t, int, int, int, int, int, T, const T*, int, const T*, int, \
+                        T, T*, int, GemmTile);                                                 \
+  template void gemm_strided<T>(hipStream_t, int, int, int, int, int, T, const T*, int,        \
+                                long long, const T*, int, long long, T, T*, int, long long,    \
+                                int, GemmTile);                                                \
+  template void gemm_tensor<T>(hipStream_t, int, int, int, T, const T*, View2, bool, bool,     \
+                               const T*, View2, bool, bool, T, T*, View2, GemmTile);
+DMRG_INST(double)
+DMRG_INST(cdouble)
+DMRG_INST(float)
+#undef DMRG_INST
+
+}  // namespace dmrg

@@ -1,0 +1,60 @@
+// Hand-written MFMA GEMM for gfx950.
+//
+//   C[M x N] = alpha * op(A) * op(B) + beta * C
+//
+// Replaces the rocBLAS dgemm / zgemm / gemm_batched calls of the reference
+// (gpu-rocm/common/scalar_traits.h:46-199, :204-406).  FP64 runs on
+// v_mfma_f64_16x16x4_f64; complex128 is four real MFMAs per k-step on separate
+// real/imaginary LDS planes; FP32 runs on v_mfma_f32_16x16x4_f32.
+//
+// Two addressing modes:
+//  * plain: column-major with leading dimensions (BLAS semantics);
+//  * two-level ("tensor") addressing, where a row or column index i of an
+//    operand is split as i = i1 + n1*i2 and lives at i1*s1 + i2*s2.  This lets
+//    one launch contract a pair of tensor legs against another pair without a
+//    permutation pass (used for the MPO-bond contraction of H_eff and of the
+//    environment updates).
+#pragma once
+#include "common.hpp"
+
+namespace dmrg {
+
+struct GemmTile {
+  int bm, bn;  // workgroup tile: 32x32, 64x64 or 128x64 (real only)
+};
+
+// Index map of one operand dimension: i -> (i % n1) * s1 + (i / n1) * s2.
+struct Dim2 {
+  int n1;
+  long long s1, s2;
+};
+// Operand view: element (r, c) at row(r) + col(c).
+struct View2 {
+  Dim2 row, col;
+};
+inline View2 view_colmajor(long long ld) {  // plain column-major matrix
+  return View2{{1 << 30, 1, 0}, {1 << 30, ld, 0}};
+}
+
+template <class T>
+void gemm(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const T* A, int lda,
+          const T* B, int ldb, T beta, T* C, int ldc, GemmTile tile = {64, 64});
+
+// Strided batch: batch b uses A + b*sA, B + b*sB, C + b*sC.
+template <class T>
+void gemm_strided(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const T* A,
+                  int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C,
+                  int ldc, long long sC, int batch, GemmTile tile = {64, 64});
+
+// Two-level-addressed GEMM.  A is viewed as (M x K), B as (K x N), C as
+// (M x N); conjA / conjB conjugate the operands.  `a_kfast` / `b_kfast` say
+// which operand index is memory-contiguous (k or m/n) to pick coalesced loads.
+template <class T>
+void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2 va, bool conjA,
+                 bool a_kfast, const T* B, View2 vb, bool conjB, bool b_kfast, T beta, T* C,
+                 View2 vc, GemmTile tile = {64, 64});
+
+// Pick a tile so that the grid covers the chip.
+GemmTile gemm_pick_tile(int M, int N, int K, bool complex_);
+
+}  // namespace dmrg

@@ -1,0 +1,569 @@
+// Lanczos vector kernels, tridiagonal eigensolver and the device generator.
+#include "lanczos.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+namespace dmrg {
+namespace {
+
+constexpr int NT = 256;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Sum over the workgroup; result valid in thread 0.
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  return r;
+}
+
+template <class T> struct Acc;
+template <> struct Acc<double> {
+  static constexpr int N = 1;
+  __device__ static void put(double v, double* o) { o[0] = v; }
+  __device__ static double get(const double* o) { return o[0]; }
+};
+template <> struct Acc<cdouble> {
+  static constexpr int N = 2;
+  __device__ static void put(cdouble v, double* o) { o[0] = v.x; o[1] = v.y; }
+  __device__ static cdouble get(const double* o) { return cdouble(o[0], o[1]); }
+};
+
+// Last-workgroup ticket: true in exactly one workgroup, after all others have
+// published their partials.
+__device__ inline bool last_block(unsigned* ticket) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(ticket, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+__device__ inline void reset_ticket(unsigned* ticket) {
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+}
+
+__device__ inline void chunk_range(size_t n, size_t& b, size_t& e) {
+  size_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk = (chunk + NT - 1) / NT * NT;
+  b = blockIdx.x * chunk;
+  e = b + chunk < n ? b + chunk : n;
+}
+
+// Sum `cnt` partial vectors of length nb (layout part[j*nb + blk]) into out[j]
+// (doubles, stride-1), one wave per j.
+__device__ inline void reduce_parts(const double* part, int nb, int cnt, double* out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int j = w; j < cnt; j += nw) {
+    double s = 0.0;
+    for (int b = lane; b < nb; b += 64) s += part[(size_t)j * nb + b];
+    s = wave_sum(s);
+    if (lane == 0) out[j] = s;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_norm(const T* __restrict__ x, size_t n, double* part,
+                                            unsigned* ticket, double* scal) {
+  __shared__ double sh[8];
+  size_t b, e;
+  chunk_range(n, b, e);
+  double acc = 0.0;
+  for (size_t i = b + threadIdx.x; i < e; i += NT) acc += abs2(x[i]);
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  if (!last_block(ticket)) return;
+  __shared__ double tot;
+  reduce_parts(part, gridDim.x, 1, &tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double nrm = sqrt(tot);
+    scal[0] = nrm;
+    scal[1] = (nrm > 1e-300) ? 1.0 / nrm : 0.0;
+  }
+  reset_ticket(ticket);
+}
+
+__device__ inline double u_pm1(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+template <class T> __device__ inline T rand_entry(uint64_t seed, uint64_t idx, double scale);
+template <> __device__ inline double rand_entry<double>(uint64_t seed, uint64_t idx, double scale) {
+  return u_pm1(seed, idx) * scale;
+}
+template <> __device__ inline cdouble rand_entry<cdouble>(uint64_t seed, uint64_t idx, double scale) {
+  return cdouble(u_pm1(seed, 2 * idx) * scale, u_pm1(seed, 2 * idx + 1) * scale);
+}
+
+template <class T>
+__global__ void k_fill_random(T* x, size_t n, uint64_t seed, uint64_t offset, double scale,
+                              const double* scal, double thr) {
+  if (scal && !(scal[0] < thr)) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = rand_entry<T>(seed, offset + i, scale);
+}
+
+template <class T>
+__global__ void k_scale(T* dst, const T* src, size_t n, const double* scal) {
+  const double s = scal[1];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i] * s;
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_alpha(const T* __restrict__ v, const T* __restrict__ w,
+                                             size_t n, double* part, unsigned* ticket,
+                                             double* alpha, int it) {
+  __shared__ double sh[8];
+  size_t b, e;
+  chunk_range(n, b, e);
+  double acc = 0.0;
+  for (size_t i = b + threadIdx.x; i < e; i += NT) acc += real(conj(v[i]) * w[i]);
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  if (!last_block(ticket)) return;
+  __shared__ double tot;
+  reduce_parts(part, gridDim.x, 1, &tot);
+  __syncthreads();
+  if (threadIdx.x == 0) alpha[it] = tot;
+  reset_ticket(ticket);
+}
+
+// w -= alpha v_it (+ beta v_{it-1}); c_j = <v_j, w> for j <= it.
+template <class T>
+__global__ __launch_bounds__(NT) void k_sub_dots(T* __restrict__ w, const T* __restrict__ V,
+                                                size_t n, int it, const double* alpha,
+                                                const double* beta, double* part,
+                                                unsigned* ticket, T* cvec) {
+  constexpr int NC = Acc<T>::N;
+  __shared__ double sh[8];
+  size_t b, e;
+  chunk_range(n, b, e);
+  const double a = alpha[it];
+  const double bm = it > 0 ? beta[it - 1] : 0.0;
+  const T* vi = V + (size_t)it * n;
+  const T* vm = it > 0 ? V + (size_t)(it - 1) * n : nullptr;
+  for (size_t i = b + threadIdx.x; i < e; i += NT) {
+    T x = w[i];
+    x = x - a * vi[i];
+    if (it > 0) x = x - bm * vm[i];
+    w[i] = x;
+  }
+  const int nb = gridDim.x;
+  for (int j0 = 0; j0 <= it; j0 += 4) {
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    for (size_t i = b + threadIdx.x; i < e; i += NT) {
+      T x = w[i];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+        if (j0 + jj <= it) acc[jj] += conj(V[(size_t)(j0 + jj) * n + i]) * x;
+    }
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      if (j0 + jj > it) break;
+      double o[2];
+      Acc<T>::put(acc[jj], o);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double s = block_sum(o[c], sh);
+        if (threadIdx.x == 0) part[((size_t)(j0 + jj) * NC + c) * nb + blockIdx.x] = s;
+      }
+    }
+  }
+  if (!last_block(ticket)) return;
+  __shared__ double red[2 * 128];
+  const int cnt = (it + 1) * NC;
+  for (int c0 = 0; c0 < cnt; c0 += 2 * 128) {
+    int cc = min(2 * 128, cnt - c0);
+    reduce_parts(part + (size_t)c0 * nb, nb, cc, red);
+    __syncthreads();
+    for (int j = threadIdx.x; j < cc; j += NT) {
+      int g = c0 + j;
+      if (NC == 1) reinterpret_cast<double*>(cvec)[g] = red[j];
+      else reinterpret_cast<double*>(cvec)[g] = red[j];  // (re, im) interleaved == cdouble layout
+    }
+    __syncthreads();
+  }
+  reset_ticket(ticket);
+}
+
+// w -= sum_j c_j v_j; beta[it] = |w|.
+template <class T>
+__global__ __launch_bounds__(NT) void k_orth_norm(T* __restrict__ w, const T* __restrict__ V,
+                                                 size_t n, int it, const T* cvec, double* part,
+                                                 unsigned* ticket, double* beta, double* scal) {
+  __shared__ double sh[8];
+  __shared__ T cs[128];
+  for (int j = threadIdx.x; j <= it; j += NT) cs[j] = cvec[j];
+  __syncthreads();
+  size_t b, e;
+  chunk_range(n, b, e);
+  double acc = 0.0;
+  for (size_t i = b + threadIdx.x; i < e; i += NT) {
+    T x = w[i];
+    for (int j = 0; j <= it; ++j) x = x - cs[j] * V[(size_t)j * n + i];
+    w[i] = x;
+    acc += abs2(x);
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  if (!last_block(ticket)) return;
+  __shared__ double tot;
+  reduce_parts(part, gridDim.x, 1, &tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bt = sqrt(tot);
+    beta[it] = bt;
+    scal[1] = (bt > 1e-300) ? 1.0 / bt : 0.0;
+  }
+  reset_ticket(ticket);
+}
+
+// out = sum_j y_j v_j; scal[0]=|out|, scal[1]=1/|out|
+template <class T>
+__global__ __launch_bounds__(NT) void k_ritz(T* __restrict__ out, const T* __restrict__ V, size_t n,
+                                            int niter, const double* y, double* part,
+                                            unsigned* ticket, double* scal) {
+  __shared__ double sh[8];
+  __shared__ double ys[128];
+  for (int j = threadIdx.x; j < niter; j += NT) ys[j] = y[j];
+  __syncthreads();
+  size_t b, e;
+  chunk_range(n, b, e);
+  double acc = 0.0;
+  for (size_t i = b + threadIdx.x; i < e; i += NT) {
+    T x = T(0);
+    for (int j = 0; j < niter; ++j) x = x + ys[j] * V[(size_t)j * n + i];
+    out[i] = x;
+    acc += abs2(x);
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  if (!last_block(ticket)) return;
+  __shared__ double tot;
+  reduce_parts(part, gridDim.x, 1, &tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double nrm = sqrt(tot);
+    scal[0] = nrm;
+    scal[1] = (nrm > 1e-300) ? 1.0 / nrm : 0.0;
+  }
+  reset_ticket(ticket);
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric tridiagonal T(k): diagonal a[0..k), off-diagonal b[0..k-1).
+// ---------------------------------------------------------------------------
+
+__device__ inline int sturm_count(const double* a, const double* b2, int k, double x, double pivmin) {
+  double q = a[0] - x;
+  int c = q < 0.0;
+  for (int i = 1; i < k; ++i) {
+    if (fabs(q) < pivmin) q = -pivmin;
+    q = (a[i] - x) - b2[i - 1] / q;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+// Lowest eigenvalue by 64-way multisection (one wave).
+__device__ double tri_lowest(const double* a, const double* b, int k, double* b2, int* cnt) {
+  const int t = threadIdx.x;
+  __shared__ double lohi[2];
+  __shared__ double pm;
+  if (t == 0) {
+    double lo = a[0], hi = a[0], mb = 0.0;
+    for (int i = 0; i < k; ++i) {
+      double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0);
+      lo = fmin(lo, a[i] - r);
+      hi = fmax(hi, a[i] + r);
+      if (i < k - 1) mb = fmax(mb, b[i] * b[i]);
+    }
+    double w = fmax(fabs(lo), fabs(hi));
+    lo -= 2.0 * DBL_EPSILON * w + DBL_MIN;
+    hi += 2.0 * DBL_EPSILON * w + DBL_MIN;
+    lohi[0] = lo;
+    lohi[1] = hi;
+    pm = DBL_MIN * fmax(1.0, mb);
+  }
+  for (int i = t; i < k - 1; i += blockDim.x) b2[i] = b[i] * b[i];
+  __syncthreads();
+  for (int round = 0; round < 24; ++round) {
+    const double lo = lohi[0], hi = lohi[1];
+    const double w = fmax(fabs(lo), fabs(hi));
+    if (hi - lo <= 2.0 * DBL_EPSILON * w + 4.0 * DBL_MIN) break;
+    const double x = lo + (hi - lo) * (double)(t + 1) / 65.0;
+    cnt[t] = sturm_count(a, b2, k, x, pm) >= 1;
+    __syncthreads();
+    if (t == 0) {
+      int l = 0;
+      while (l < 64 && !cnt[l]) ++l;
+      double nlo = (l == 0) ? lo : lo + (hi - lo) * (double)l / 65.0;
+      double nhi = (l == 64) ? hi : lo + (hi - lo) * (double)(l + 1) / 65.0;
+      lohi[0] = nlo;
+      lohi[1] = nhi;
+    }
+    __syncthreads();
+  }
+  double r = 0.5 * (lohi[0] + lohi[1]);
+  __syncthreads();
+  return r;
+}
+
+// Eigenvector of T for eigenvalue lam by inverse iteration (thread 0).
+__device__ void tri_vector(const double* a, const double* b, int k, double lam, double* y) {
+  if (k == 1) { y[0] = 1.0; return; }
+  double d[128], du[128], dl[128], du2[128], x[128];
+  int piv[128];
+  double tnorm = 0.0;
+  for (int i = 0; i < k; ++i)
+    tnorm = fmax(tnorm, fabs(a[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0));
+  const double tiny = fmax(DBL_EPSILON * tnorm, DBL_MIN);
+  for (int i = 0; i < k; ++i) {
+    d[i] = a[i] - lam;
+    if (i < k - 1) { du[i] = b[i]; dl[i] = b[i]; }
+    du2[i] = 0.0;
+  }
+  for (int i = 0; i < k - 1; ++i) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (fabs(d[i]) < tiny) d[i] = d[i] >= 0.0 ? tiny : -tiny;
+      double f = dl[i] / d[i];
+      dl[i] = f;
+      d[i + 1] -= f * du[i];
+      piv[i] = i;
+    } else {
+      double f = d[i] / dl[i];
+      d[i] = dl[i];
+      dl[i] = f;
+      double tmp = du[i];
+      du[i] = d[i + 1];
+      d[i + 1] = tmp - f * d[i + 1];
+      if (i < k - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du[i + 1];
+      }
+      piv[i] = i + 1;
+    }
+  }
+  if (fabs(d[k - 1]) < tiny) d[k - 1] = d[k - 1] >= 0.0 ? tiny : -tiny;
+  for (int i = 0; i < k; ++i) x[i] = 1.0 + 0.25 * (double)i / (double)k;
+  for (int iter = 0; iter < 3; ++iter) {
+    for (int i = 0; i < k - 1; ++i) {
+      if (piv[i] == i) {
+        x[i + 1] -= dl[i] * x[i];
+      } else {
+        double tmp = x[i];
+        x[i] = x[i + 1];
+        x[i + 1] = tmp - dl[i] * x[i];
+      }
+    }
+    x[k - 1] /= d[k - 1];
+    if (k > 1) x[k - 2] = (x[k - 2] - du[k - 2] * x[k - 1]) / d[k - 2];
+    for (int i = k - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
+    double s = 0.0, mx = 0.0;
+    for (int i = 0; i < k; ++i) s += x[i] * x[i], mx = fmax(mx, fabs(x[i]));
+    // rescale first to avoid overflow of the next solve
+    double inv = 1.0 / sqrt(s);
+    for (int i = 0; i < k; ++i) x[i] *= inv;
+    (void)mx;
+  }
+  // deterministic sign: largest-magnitude component positive
+  int im = 0;
+  for (int i = 1; i < k; ++i)
+    if (fabs(x[i]) > fabs(x[im])) im = i;
+  double sg = x[im] < 0.0 ? -1.0 : 1.0;
+  for (int i = 0; i < k; ++i) y[i] = sg * x[i];
+}
+
+__global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* beta, int k, int mode,
+                                           double beta_tol, double eig_tol, double* scal,
+                                           int* status, double* y) {
+  __shared__ double b2[128];
+  __shared__ int cnt[64];
+  __shared__ int bidx;
+  const int t = threadIdx.x;
+  if (mode == 0) {  // check
+    if (t == 0) {
+      bidx = -1;
+      for (int j = 0; j < k; ++j)
+        if (beta[j] < beta_tol) { bidx = j; break; }
+    }
+    __syncthreads();
+    if (bidx >= 0) {
+      if (t == 0) { status[0] = 1; status[1] = bidx + 1; }
+      return;
+    }
+    double lam = tri_lowest(alpha, beta, k, b2, cnt);
+    if (t == 0) {
+      scal[4] = lam;
+      double prev = scal[2];
+      if (fabs(lam - prev) < eig_tol) { status[0] = 1; status[1] = k; }
+      else { status[0] = 0; scal[2] = lam; }
+    }
+  } else {  // final
+    double lam = (k == 1) ? alpha[0] : tri_lowest(alpha, beta, k, b2, cnt);
+    if (t == 0) {
+      scal[3] = lam;
+      tri_vector(alpha, beta, k, lam, y);
+    }
+  }
+}
+
+__global__ void k_lz_reset(double* scal, int* status) {
+  scal[2] = 1e30;
+  status[0] = 0;
+  status[1] = 0;
+}
+
+}  // namespace
+
+int lz_blocks(size_t n) {
+  size_t nb = (n + 2047) / 2048;
+  if (nb < 1) nb = 1;
+  if (nb > 1024) nb = 1024;
+  return (int)nb;
+}
+
+void LzDev::alloc(int kmax_, size_t part_bytes_, size_t elem_bytes) {
+  release();
+  kmax = kmax_;
+  part_bytes = part_bytes_;
+  DMRG_HIP(hipMalloc(&alpha, kmax * sizeof(double)));
+  DMRG_HIP(hipMalloc(&beta, kmax * sizeof(double)));
+  DMRG_HIP(hipMalloc(&y, kmax * sizeof(double)));
+  DMRG_HIP(hipMalloc(&scal, 8 * sizeof(double)));
+  DMRG_HIP(hipMalloc(&status, 4 * sizeof(int)));
+  DMRG_HIP(hipMalloc(&part, part_bytes));
+  DMRG_HIP(hipMalloc(&ticket, 4 * sizeof(unsigned)));
+  DMRG_HIP(hipMalloc(&cvec, kmax * elem_bytes));
+  DMRG_HIP(hipMemset(ticket, 0, 4 * sizeof(unsigned)));
+  DMRG_HIP(hipMemset(alpha, 0, kmax * sizeof(double)));
+  DMRG_HIP(hipMemset(beta, 0, kmax * sizeof(double)));
+}
+
+void LzDev::release() {
+  for (void* p : {(void*)alpha, (void*)beta, (void*)y, (void*)scal, (void*)status, part,
+                  (void*)ticket, cvec})
+    if (p) (void)hipFree(p);
+  alpha = beta = y = scal = nullptr;
+  status = nullptr;
+  part = cvec = nullptr;
+  ticket = nullptr;
+}
+
+template <class T>
+void lz_norm(hipStream_t st, LzDev& d, const T* x, size_t n) {
+  hipLaunchKernelGGL(k_norm<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, x, n, (double*)d.part,
+                     d.ticket, d.scal);
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void lz_fill_if_small(hipStream_t st, LzDev& d, T* x, size_t n, uint64_t seed, double thr) {
+  hipLaunchKernelGGL(k_fill_random<T>, dim3(std::min<size_t>(cdiv(n, NT), 2048)), dim3(NT), 0, st,
+                     x, n, seed, 0ull, 1.0, d.scal, thr);
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void fill_random(hipStream_t st, T* x, size_t n, uint64_t seed, uint64_t offset, double scale) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_random<T>, dim3(std::min<size_t>(cdiv(n, NT), 2048)), dim3(NT), 0, st,
+                     x, n, seed, offset, scale, (const double*)nullptr, 0.0);
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void lz_scale(hipStream_t st, LzDev& d, T* dst, const T* src, size_t n) {
+  hipLaunchKernelGGL(k_scale<T>, dim3(std::min<size_t>(cdiv(n, NT), 2048)), dim3(NT), 0, st, dst,
+                     src, n, d.scal);
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void lz_alpha(hipStream_t st, LzDev& d, const T* v, const T* w, size_t n, int it) {
+  hipLaunchKernelGGL(k_alpha<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, v, w, n, (double*)d.part,
+                     d.ticket, d.alpha, it);
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void lz_sub_dots(hipStream_t st, LzDev& d, T* w, const T* V, size_t n, int it) {
+  DMRG_REQUIRE(it < d.kmax && it < 128, "lanczos: Krylov dimension exceeds 128");
+  size_t need = (size_t)(it + 1) * Acc<T>::N * lz_blocks(n) * sizeof(double);
+  DMRG_REQUIRE(need <= d.part_bytes, "lanczos: partial buffer too small");
+  hipLaunchKernelGGL(k_sub_dots<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, w, V, n, it, d.alpha,
+                     d.beta, (double*)d.part, d.ticket, (T*)d.cvec);
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void lz_orth_norm(hipStream_t st, LzDev& d, T* w, const T* V, size_t n, int it) {
+  hipLaunchKernelGGL(k_orth_norm<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, w, V, n, it,
+                     (const T*)d.cvec, (double*)d.part, d.ticket, d.beta, d.scal);
+  DMRG_LAUNCH_CHECK();
+}
+
+void lz_check(hipStream_t st, LzDev& d, int ncheck, double beta_tol, double eig_tol) {
+  DMRG_REQUIRE(ncheck <= 128, "lanczos: tridiagonal larger than 128");
+  hipLaunchKernelGGL(k_tri, dim3(1), dim3(64), 0, st, d.alpha, d.beta, ncheck, 0, beta_tol, eig_tol,
+                     d.scal, d.status, d.y);
+  DMRG_LAUNCH_CHECK();
+}
+
+void lz_final(hipStream_t st, LzDev& d, int niter) {
+  DMRG_REQUIRE(niter >= 1 && niter <= 128, "lanczos: bad Krylov dimension");
+  hipLaunchKernelGGL(k_tri, dim3(1), dim3(64), 0, st, d.alpha, d.beta, niter, 1, 0.0, 0.0, d.scal,
+                     d.status, d.y);
+  DMRG_LAUNCH_CHECK();
+}
+
+void lz_reset(hipStream_t st, LzDev& d) {
+  hipLaunchKernelGGL(k_lz_reset, dim3(1), dim3(1), 0, st, d.scal, d.status);
+  DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void lz_ritz(hipStream_t st, LzDev& d, T* out, const T* V, size_t n, int niter) {
+  DMRG_REQUIRE(niter <= 128, "lanczos: Krylov dimension exceeds 128");
+  hipLaunchKernelGGL(k_ritz<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, out, V, n, niter, d.y,
+                     (double*)d.part, d.ticket, d.scal);
+  DMRG_LAUNCH_CHECK();
+}
+
+#define DMRG_INST(T)                                                                        \
+  template void lz_norm<T>(hipStream_t, LzDev&, const T*, size_t);                          \
+  template void lz_fill_if_small<T>(hipStream_t, LzDev&, T*, size_t, uint64_t, double);     \
+  template void lz_scale<T>(hipStream_t, LzDev&, T*, const T*, size_t);                     \
+  template void lz_alpha<T>(hipStream_t, LzDev&, const T*, const T*, size_t, int);          \
+  template void lz_sub_dots<T>(hipStream_t, LzDev&, T*, const T*, size_t, int);             \
+  template void lz_orth_norm<T>(hipStream_t, LzDev&, T*, const T*, size_t, int);            \
+  template void lz_ritz<T>(hipStream_t, LzDev&, T*, const T*, size_t, int);                 \
+  template void fill_random<T>(hipStream_t, T*, size_t, uint64_t, uint64_t, double);
+DMRG_INST(double)
+DMRG_INST(cdouble)
+#undef DMRG_INST
+
+}  // namespace dmrg

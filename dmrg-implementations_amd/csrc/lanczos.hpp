@@ -1,0 +1,60 @@
+// Lanczos building blocks (replaces the rocBLAS dot/axpy/gemv/nrm2/scal
+// sequence and the rocsolver_dsteqr calls of the reference's
+// lanczos_eigensolver, gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h:970-1160, and
+// the scalar kernels of gpu-rocm/common/scalar_traits.h:422-489).
+//
+// Every reduction is a single launch: each workgroup writes a partial and the
+// last workgroup to finish (device-scope atomic ticket) reduces the partials
+// in a fixed order, so results are deterministic.
+#pragma once
+#include "common.hpp"
+
+namespace dmrg {
+
+struct LzDev {
+  // device scalars / small arrays
+  double* alpha = nullptr;     // [kmax]
+  double* beta = nullptr;      // [kmax]
+  double* y = nullptr;         // [kmax] Ritz coefficients
+  double* scal = nullptr;      // [8]: 0 norm, 1 inv, 2 prev_energy, 3 energy, 4 check value
+  int* status = nullptr;       // [4]: 0 stop flag, 1 niter
+  void* part = nullptr;        // partial sums (bytes)
+  unsigned* ticket = nullptr;  // last-block counters [4]
+  void* cvec = nullptr;        // [kmax] projection coefficients (T)
+  int kmax = 0;
+  size_t part_bytes = 0;
+  void alloc(int kmax_, size_t part_bytes_, size_t elem_bytes);
+  void release();
+};
+
+// Number of workgroups used by the vector kernels for length n.
+int lz_blocks(size_t n);
+
+// scal[0] = |x|, scal[1] = 1/|x| (0 when |x| <= 1e-300).
+template <class T> void lz_norm(hipStream_t st, LzDev& d, const T* x, size_t n);
+// If scal[0] < thr, overwrite x with the splitmix64 stream `seed` (U[-1,1)).
+template <class T> void lz_fill_if_small(hipStream_t st, LzDev& d, T* x, size_t n, uint64_t seed,
+                                         double thr);
+// dst = src * scal[1]
+template <class T> void lz_scale(hipStream_t st, LzDev& d, T* dst, const T* src, size_t n);
+// alpha[it] = Re <v, w>
+template <class T> void lz_alpha(hipStream_t st, LzDev& d, const T* v, const T* w, size_t n, int it);
+// w -= alpha[it] v_it + beta[it-1] v_{it-1};  c_j = <v_j, w>, j <= it
+template <class T> void lz_sub_dots(hipStream_t st, LzDev& d, T* w, const T* V, size_t n, int it);
+// w -= sum_j c_j v_j;  beta[it] = |w|, scal[1] = 1/beta (0 if beta <= 1e-300)
+template <class T> void lz_orth_norm(hipStream_t st, LzDev& d, T* w, const T* V, size_t n, int it);
+// prev_energy := 1e30, status := 0 (start of a Lanczos call).
+void lz_reset(hipStream_t st, LzDev& d);
+// Convergence check on T(ncheck) (every third iteration).
+void lz_check(hipStream_t st, LzDev& d, int ncheck, double beta_tol, double eig_tol);
+// Lowest eigenpair of T(niter) -> scal[3], y.
+void lz_final(hipStream_t st, LzDev& d, int niter);
+// out = V y (niter columns), scal[0]=|out|, scal[1]=1/|out|
+template <class T> void lz_ritz(hipStream_t st, LzDev& d, T* out, const T* V, size_t n, int niter);
+
+// Fill `x` with entries scale*U[-1,1) from the splitmix64 stream `seed`,
+// starting at stream position `offset` (complex: two outputs per entry).
+template <class T> void fill_random(hipStream_t st, T* x, size_t n, uint64_t seed, uint64_t offset,
+                                    double scale);
+
+}  // namespace dmrg

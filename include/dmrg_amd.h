@@ -1,0 +1,207 @@
+/*
+ * dmrg_amd.h -- C-ABI of the MI355X (gfx950) DMRG sweep engine.
+ *
+ * Drop-in for the single-GPU sweep path of the reference
+ * (gpu-rocm/dmrg2-gpu DMRG2GPU<Scalar> and gpu-rocm/dmrg-gpu DMRGGPU<Scalar>):
+ * an MPO and an initial MPS go in, the ground-state energy and the truncated
+ * MPS come out.  Plain pointers and sizes only; no exception crosses the ABI.
+ *
+ * Layouts (all column-major, exactly the reference's device layouts):
+ *   MPO site tensor  W[w + s*D + sp*D*d + wp*D*d*d]   s = ket, sp = bra,
+ *                    left boundary row w = 0, right boundary column wp = D-1
+ *                    (gpu-rocm/dmrg2-gpu/src/test_dmrg2_gpu.cpp:22-34)
+ *   MPS site tensor  A[a + s*cL + b*cL*d]                 shape (cL, d, cR)
+ *   Complex scalars are interleaved (re, im) doubles (hipDoubleComplex).
+ *
+ * Ownership: the caller owns every host buffer; the handle owns all device
+ * memory.  No pointer is retained past the call that receives it.
+ * Threading: one handle per host thread; a handle is not re-entrant.
+ */
+#ifndef DMRG_AMD_H
+#define DMRG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference throws std::runtime_error from HIP_CHECK /
+ * ROCBLAS_CHECK (gpu-rocm/common/hip_check.h:13-31) and on dsteqr failure
+ * (gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h:1120-1122); here each failure is a
+ * return code with the message in dmrg_last_error(). */
+enum {
+  DMRG_OK = 0,
+  DMRG_ERR_HIP = 1,             /* HIP runtime / launch failure          */
+  DMRG_ERR_SOLVER_NO_CONV = 2,  /* SVD or eigensolver did not converge   */
+  DMRG_ERR_BAD_ARG = 3,         /* invalid argument or shape             */
+  DMRG_ERR_NONFINITE = 4,       /* NaN / Inf met in the sweep            */
+  DMRG_ERR_STATE = 5,           /* call out of order (e.g. run before set_mpo) */
+  DMRG_ERR_OOM = 6              /* device allocation failed              */
+};
+
+enum { DMRG_DTYPE_F64 = 0, DMRG_DTYPE_C128 = 1 };
+enum { DMRG_SINGLE_SITE = 1, DMRG_TWO_SITE = 2 };
+/* Arithmetic of the H_eff matvec.  Environments, Lanczos (tridiagonal solve,
+ * reorthogonalisation) and the truncation SVD are FP64 in every mode. */
+enum { DMRG_PREC_FP64 = 0, DMRG_PREC_FP32 = 1 };
+
+typedef struct dmrg_options {
+  int krylov_max;        /* Lanczos Krylov dimension cap; reference: min(100, n)
+                            (dmrg2_gpu_impl.h:138, :974).  <= 128.            */
+  double lanczos_tol;    /* beta breakdown threshold, reference 1e-12 (:975) */
+  double eig_tol;        /* Ritz-value convergence threshold, 1e-12 (:976)   */
+  double svd_cutoff;     /* absolute singular-value cutoff, 1e-14 (:1276)    */
+  double svd_rel_cutoff; /* relative cutoff (sigma_i < rel*sigma_0); 0 = off */
+  int precision;         /* DMRG_PREC_*                                       */
+  int profile;           /* 1: record per-phase HIP-event timings             */
+  int device;            /* HIP device ordinal                                */
+  int svd_max_sweeps;    /* Jacobi sweep cap of the SVD                       */
+} dmrg_options;
+
+typedef struct dmrg_stats {
+  double lanczos_ms;     /* host-measured phase times (sum over bonds)      */
+  double heff_ms;        /* HIP-event time of all H_eff matvecs on the stream */
+  double svd_ms;
+  double env_ms;
+  double sweep_s_last;   /* wall seconds of the last completed sweep        */
+  double heff_flop;      /* algorithmic H_eff flops (F_heff per matvec)     */
+  double env_flop;
+  long long matvecs;
+  long long lanczos_calls;
+  long long svd_calls;
+  long long svd_sweeps;
+  int sweeps_done;
+  double max_discarded;  /* largest truncated weight sum(sigma_i^2) of a bond */
+  double env_build_s;    /* wall seconds of the last environment build      */
+  double sweeps_s_total; /* wall seconds of all sweeps (environment build excluded) */
+} dmrg_stats;
+
+typedef struct dmrg_engine* dmrg_handle;
+
+/* Fill `opt` with the reference's hard-coded defaults. */
+void dmrg_default_options(dmrg_options* opt);
+
+/* Replaces DMRG2GPU(int L, int d, int chi_max, int D_mpo, double tol)
+ * (gpu-rocm/dmrg2-gpu/src/dmrg2_gpu.h:30; impl :51-227) and
+ * DMRGGPU(...) (gpu-rocm/dmrg-gpu/src/dmrg_gpu.h:30).  Bond dimensions start
+ * at min(d^min(i, L-i), chi_max) (dmrg2_gpu_impl.h:70-76).  opt may be NULL. */
+int dmrg_create(int L, int d, int chi_max, int D, double tol, int dtype, int variant,
+                const dmrg_options* opt, dmrg_handle* out);
+/* Replaces ~DMRG2GPU (dmrg2_gpu_impl.h:266-343).  NULL is ignored. */
+void dmrg_destroy(dmrg_handle h);
+
+/* Replaces set_mpo(const std::vector<Scalar*>&) (dmrg2_gpu.h:38; impl :435-576):
+ * W[i] points to the D*d*d*D scalars of site i. */
+int dmrg_set_mpo(dmrg_handle h, const void* const* W);
+
+/* Replaces initialize_mps_random(double scale) (dmrg2_gpu_impl.h:382-393).
+ * Entries scale*U[-1,1) from the counter-based splitmix64 stream `seed`
+ * (one stream over all sites, Fortran order, complex: re then im); the
+ * reference's unseeded rand() is not reproducible. */
+int dmrg_init_mps_random(dmrg_handle h, uint64_t seed, double scale);
+/* Replaces initialize_mps_product() (dmrg2_gpu_impl.h:396-413). */
+int dmrg_init_mps_product(dmrg_handle h);
+/* Replaces initialize_mps_neel() (dmrg2_gpu_impl.h:415-429). */
+int dmrg_init_mps_neel(dmrg_handle h);
+/* Set one site tensor (cL, d, cR); the bond dimensions of the neighbouring
+ * bonds become cL and cR.  No reference counterpart (the reference only has
+ * the initialize_mps_* generators); used to feed identical states to the
+ * engine and to a CPU path, and to load the reference's .bin/.npz files. */
+int dmrg_set_mps(dmrg_handle h, int site, const void* A, int cL, int cR);
+
+/* Replaces double run(int n_sweeps) (dmrg2_gpu_impl.h:1426-1457): build the
+ * environments, then alternate L->R and R->L sweeps until |E - E_prev| < tol
+ * after the second sweep.  energy may be NULL.  When `quiet` is 0 it prints
+ * the reference's stdout contract ("Final energy: %.12f", "Total wall time:
+ * %.3f s", "  env_build_sec: %.3f  timer_scope: sweep_only"). */
+int dmrg_run(dmrg_handle h, int n_sweeps, int quiet, double* energy);
+
+/* Replaces get_energy() (dmrg2_gpu.h:40). */
+int dmrg_get_energy(dmrg_handle h, double* energy);
+/* Energies at the end of each completed sweep (returned count in *n). */
+int dmrg_get_sweep_energies(dmrg_handle h, double* out, int cap, int* n);
+/* Replaces chi_L / chi_R (dmrg2_gpu.h:42-43): dims[0..L] of all bonds. */
+int dmrg_get_bond_dims(dmrg_handle h, int* dims);
+/* Replaces get_mps (dmrg2_gpu_impl.h:1463-1472) for one site: copies
+ * cL*d*cR scalars to `out`. */
+int dmrg_get_mps(dmrg_handle h, int site, void* out);
+/* Copy environment `idx` (left: L[idx], sites 0..idx-1 contracted; right:
+ * R[idx], sites idx..L-1) of shape (chi, D, chi), layout (ket, mpo, bra), to
+ * `out` (NULL: only report chi).  The reference keeps d_L_envs_/d_R_envs_
+ * private (dmrg2_gpu.h:60-61); exposed here for checks and the CPU baseline. */
+int dmrg_get_env(dmrg_handle h, int left, int idx, void* out, int* chi);
+/* FP64 <psi|H|psi> / <psi|psi> of the current MPS by a full environment
+ * contraction.  No reference counterpart; used to check a returned state. */
+int dmrg_expectation(dmrg_handle h, double* energy, double* norm2);
+/* Per-phase timings and counters; no reference counterpart beyond the
+ * DMRG_GPU_PROFILE phase timers (gpu-rocm/common/gpu_opts.h:57-93). */
+int dmrg_get_stats(dmrg_handle h, dmrg_stats* st);
+int dmrg_reset_stats(dmrg_handle h);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* dmrg_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Operator-level entry points on device buffers (for parity tests and the
+ * roofline bench).  Each replaces one library-call sequence of the reference
+ * and runs on the given HIP stream (0 = default stream); `ctx` owns scratch.
+ * ---------------------------------------------------------------------- */
+typedef struct dmrg_ctx* dmrg_ctx_handle;
+int dmrg_ctx_create(int device, dmrg_ctx_handle* out);
+void dmrg_ctx_destroy(dmrg_ctx_handle c);
+
+/* Two-site H_eff matvec: apply_heff_two_site (dmrg2_gpu_impl.h:606-780).
+ * L (cL,D,cL), R (cR,D,cR), W1/W2 raw MPO tensors of the two sites (device),
+ * theta/out (cL,d,d,cR).  dtype: DMRG_DTYPE_*. */
+int dmrg_op_heff2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, int d, int D,
+                  const void* L, const void* R, const void* W1, const void* W2,
+                  const void* theta, void* out);
+/* Single-site matvec: DMRGGPU::apply_heff (dmrg_gpu_impl.h:538-705). */
+int dmrg_op_heff1(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, int d, int D,
+                  const void* L, const void* R, const void* W, const void* theta, void* out);
+/* update_left_env (dmrg2_gpu_impl.h:786-860): Lnew (co,D,co). */
+int dmrg_op_env_left(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co, int d, int D,
+                     const void* L, const void* A, const void* W, void* Lnew);
+/* update_right_env (dmrg2_gpu_impl.h:866-934): A (co,d,ci), R (ci,D,ci). */
+int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co, int d, int D,
+                      const void* R, const void* A, const void* W, void* Rnew);
+/* Truncated SVD split of svd_split (dmrg2_gpu_impl.h:1166-1343): M (m x n),
+ * keep k <= min(m,n) values, new rank by the cutoffs; left (m x nk) = U or
+ * U*S, right (nk x n) = V^H or S*V^H (scale_left selects); sigma gets the nk
+ * kept values (device, may be NULL).  *nk_out on the host. */
+int dmrg_op_svd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, int k,
+                      double cutoff, double rel_cutoff, int scale_left, const void* M,
+                      void* left, void* right, double* sigma, int* nk_out);
+/* Lowest eigenpair of the symmetric tridiagonal (alpha[0..k), beta[0..k-1))
+ * on the device -- replaces rocsolver_dsteqr (dmrg2_gpu_impl.h:1110-1125).
+ * Host arrays in, host results out. */
+int dmrg_op_tridiag_lowest(dmrg_ctx_handle c, int k, const double* alpha, const double* beta,
+                           double* lam, double* y);
+/* One Lanczos solve on the two-site H_eff (lanczos_eigensolver,
+ * dmrg2_gpu_impl.h:970-1160).  theta is overwritten with the Ritz vector. */
+int dmrg_op_lanczos2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, int d, int D,
+                     const void* L, const void* R, const void* W1, const void* W2, void* theta,
+                     int krylov_max, int site, double* energy, int* niter);
+
+/* C = alpha*op(A)*op(B) + beta*C on device buffers (column-major; op: 0 N,
+ * 1 T, 2 conj, 3 conj-transpose).  Replaces the Traits::gemm dispatch of
+ * gpu-rocm/common/scalar_traits.h:46-58 (rocblas_dgemm / rocblas_zgemm).
+ * alpha/beta point to one scalar of the dtype (host memory). */
+int dmrg_op_gemm(dmrg_ctx_handle c, void* stream, int dtype, int opA, int opB, int M, int N,
+                 int K, const void* alpha, const void* A, int lda, const void* B, int ldb,
+                 const void* beta, void* Cm, int ldc);
+
+/* Device-memory helpers so callers need no other HIP binding. */
+int dmrg_dev_malloc(void** p, size_t bytes);
+int dmrg_dev_free(void* p);
+int dmrg_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int dmrg_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int dmrg_dev_synchronize(void);
+int dmrg_device_count(int* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMRG_AMD_H */

@@ -1,0 +1,174 @@
+"""Operator parity on the GPU: each C-ABI op against the numpy oracle.
+
+Tolerances are relative Frobenius errors; FP64 accumulations of K <= 5120
+terms stay below ~1e-14 relative, so 1e-12 leaves margin without hiding a
+wrong index (which gives O(1) errors).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import rand, rel_err
+from oracle import dmrg as od
+from oracle import models
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _dev(dmrg_amd, a):
+    return dmrg_amd.DeviceArray.from_numpy(a)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("opA,opB", [(0, 0), (1, 0), (0, 1), (1, 1), (3, 0), (0, 3), (2, 2), (3, 3)])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (16, 16, 4), (37, 53, 29), (130, 70, 260), (300, 257, 64)])
+def test_gemm(dmrg_amd, ops, cplx, opA, opB, M, N, K):
+    if not cplx and opA in (2, 3):
+        opA -= 2
+    if not cplx and opB in (2, 3):
+        opB -= 2
+    rng = np.random.default_rng(M * 1000 + N * 10 + K + 7 * opA + 3 * opB)
+    A = rand(rng, (K, M) if opA in (1, 3) else (M, K), cplx)
+    B = rand(rng, (N, K) if opB in (1, 3) else (K, N), cplx)
+    C0 = rand(rng, (M, N), cplx)
+    f = {0: lambda x: x, 1: lambda x: x.T, 2: lambda x: x.conj(), 3: lambda x: x.conj().T}
+    alpha, beta = (0.7 - 0.2j, 0.3 + 0.1j) if cplx else (0.7, 0.3)
+    ref = alpha * f[opA](A) @ f[opB](B) + beta * C0
+    dA, dB, dC = _dev(dmrg_amd, A), _dev(dmrg_amd, B), _dev(dmrg_amd, C0)
+    ops.gemm(opA, opB, dA, dB, dC, alpha, beta)
+    got = dC.numpy()
+    assert rel_err(got, ref) < TOL, (rel_err(got, ref), got.ravel()[:4], ref.ravel()[:4])
+
+
+def _rand_problem(rng, cL, cR, d, D, cplx):
+    L = rand(rng, (cL, D, cL), cplx)
+    R = rand(rng, (cR, D, cR), cplx)
+    W1 = rand(rng, (D, d, d, D), cplx)
+    W2 = rand(rng, (D, d, d, D), cplx)
+    th = rand(rng, (cL, d, d, cR), cplx)
+    return L, R, W1, W2, th
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("cL,cR,d,D", [(1, 2, 2, 5), (3, 4, 2, 5), (16, 16, 2, 5), (33, 20, 2, 5),
+                                       (64, 64, 2, 5), (12, 9, 5, 4), (40, 40, 3, 4)])
+def test_heff2(dmrg_amd, ops, cplx, cL, cR, d, D):
+    rng = np.random.default_rng(cL * 97 + cR + d + D)
+    L, R, W1, W2, th = _rand_problem(rng, cL, cR, d, D, cplx)
+    ref = od.apply_heff2(L, R, W1, W2, th)
+    ein = np.einsum("cwa,csre,wspm,mrqn,enf->apqf", L, th, W1, W2, R, optimize=True)
+    assert rel_err(ref, ein) < 1e-13
+    out = dmrg_amd.DeviceArray(th.shape, th.dtype)
+    ops.heff2(_dev(dmrg_amd, L), _dev(dmrg_amd, R), _dev(dmrg_amd, W1), _dev(dmrg_amd, W2),
+              _dev(dmrg_amd, th), out)
+    assert rel_err(out.numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("cL,cR,d,D", [(1, 2, 2, 5), (5, 7, 2, 5), (32, 32, 2, 5), (20, 25, 5, 4)])
+def test_heff1(dmrg_amd, ops, cplx, cL, cR, d, D):
+    rng = np.random.default_rng(cL * 31 + cR + d * D)
+    L = rand(rng, (cL, D, cL), cplx)
+    R = rand(rng, (cR, D, cR), cplx)
+    W = rand(rng, (D, d, d, D), cplx)
+    th = rand(rng, (cL, d, cR), cplx)
+    ref = od.apply_heff1(L, R, W, th)
+    out = dmrg_amd.DeviceArray(th.shape, th.dtype)
+    ops.heff1(_dev(dmrg_amd, L), _dev(dmrg_amd, R), _dev(dmrg_amd, W), _dev(dmrg_amd, th), out)
+    assert rel_err(out.numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("ci,co,d,D", [(1, 2, 2, 5), (3, 4, 2, 5), (30, 45, 2, 5), (64, 64, 2, 5),
+                                       (10, 13, 5, 4)])
+def test_env_updates(dmrg_amd, ops, cplx, ci, co, d, D):
+    rng = np.random.default_rng(ci * 13 + co + d + D)
+    Lg = rand(rng, (ci, D, ci), cplx)
+    A = rand(rng, (ci, d, co), cplx)
+    W = rand(rng, (D, d, d, D), cplx)
+    refL = od.update_left_env(Lg, A, W)
+    out = dmrg_amd.DeviceArray((co, D, co), A.dtype)
+    ops.env_left(_dev(dmrg_amd, Lg), _dev(dmrg_amd, A), _dev(dmrg_amd, W), out)
+    assert rel_err(out.numpy(), refL) < TOL
+    Rg = rand(rng, (ci, D, ci), cplx)
+    B = rand(rng, (co, d, ci), cplx)
+    refR = od.update_right_env(Rg, B, W)
+    outR = dmrg_amd.DeviceArray((co, D, co), A.dtype)
+    ops.env_right(_dev(dmrg_amd, Rg), _dev(dmrg_amd, B), _dev(dmrg_amd, W), outR)
+    assert rel_err(outR.numpy(), refR) < TOL
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("m,n", [(4, 4), (1, 8), (8, 1), (30, 17), (17, 30), (64, 64), (130, 96),
+                                 (256, 256)])
+def test_svd_split(dmrg_amd, ops, cplx, m, n):
+    rng = np.random.default_rng(m * 7 + n)
+    M = rand(rng, (m, n), cplx)
+    k = min(m, n)
+    left, right, sig, nk = ops.svd_split(_dev(dmrg_amd, M), k)
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    assert nk == k
+    # LAPACK-class bound p(n) eps sigma_max with p(n) ~ n
+    assert np.max(np.abs(sig - s_ref[:nk])) < 1e-12 * s_ref[0]
+    # U orthonormal; right = S V^H, so U (S V^H) = M
+    assert np.linalg.norm(left.conj().T @ left - np.eye(nk)) < 1e-12
+    assert rel_err(left @ right, M) < 1e-12
+    vh = right / sig[:, None]
+    assert np.linalg.norm(vh @ vh.conj().T - np.eye(nk)) < 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_svd_truncation_rank(dmrg_amd, ops, cplx):
+    rng = np.random.default_rng(5)
+    m, n, r = 60, 48, 20
+    U = np.linalg.qr(rand(rng, (m, m), cplx))[0][:, :r]
+    V = np.linalg.qr(rand(rng, (n, n), cplx))[0][:, :r]
+    s = np.logspace(0, -9, r)
+    M = (U * s) @ V.conj().T
+    left, right, sig, nk = ops.svd_split(_dev(dmrg_amd, M), 40, scale_left=True)
+    assert nk == r  # values beyond the rank fall under the 1e-14 cutoff
+    assert np.max(np.abs(sig - s) / s) < 1e-8  # relative accuracy down to 1e-9
+    assert rel_err(left @ right, M) < 1e-12
+    assert np.linalg.norm(right @ right.conj().T - np.eye(nk)) < 1e-12
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 7, 20, 64, 100])
+def test_tridiag_lowest(ops, k):
+    rng = np.random.default_rng(k)
+    a = rng.uniform(-2, 2, k)
+    b = rng.uniform(0.1, 1.0, max(k - 1, 0))
+    lam, y = ops.tridiag_lowest(a, b)
+    if k == 1:
+        assert lam == a[0]
+        return
+    ev, evec = sla.eigh_tridiagonal(a, b)
+    assert abs(lam - ev[0]) < 1e-14 * max(1, abs(ev[0]))
+    assert min(np.linalg.norm(y - evec[:, 0]), np.linalg.norm(y + evec[:, 0])) < 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_lanczos2_matches_oracle(dmrg_amd, ops, cplx):
+    rng = np.random.default_rng(11)
+    cL, cR, d, D = 8, 8, 2, 5
+    # Hermitian H_eff: environments from a random MPS of the Heisenberg chain
+    Ws = models.heisenberg_mpo(6) if not cplx else models.josephson_mpo(6, n_max=1)
+    d = Ws[0].shape[1]
+    D = Ws[0].shape[0]
+    mps, b = od.random_mps(6, d, 8, seed=3, complex_=cplx)
+    dt = complex if cplx else float
+    Lg = od.left_boundary(D, dt)
+    for i in range(2):
+        Lg = od.update_left_env(Lg, mps[i], Ws[i])
+    Rg = od.right_boundary(D, dt)
+    for i in range(5, 3, -1):
+        Rg = od.update_right_env(Rg, mps[i], Ws[i])
+    th = np.einsum("asm,mtb->astb", mps[2], mps[3])
+    E_ref, psi_ref = od.lanczos(lambda x: od.apply_heff2(Lg, Rg, Ws[2], Ws[3], x), th, 2, 20)
+    dth = _dev(dmrg_amd, th)
+    E, nit = ops.lanczos2(_dev(dmrg_amd, Lg), _dev(dmrg_amd, Rg), _dev(dmrg_amd, Ws[2]),
+                          _dev(dmrg_amd, Ws[3]), dth, krylov_max=20, site=2)
+    assert abs(E - E_ref) < 1e-11 * max(1, abs(E_ref))
+    psi = dth.numpy()
+    ov = abs(np.vdot(psi_ref.ravel(), psi.ravel()))
+    assert abs(ov - 1) < 1e-9
