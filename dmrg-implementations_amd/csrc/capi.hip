@@ -47,6 +47,7 @@ Options to_opts(const dmrg_options* o) {
     r.profile = o->profile;
     r.device = o->device;
     r.svd_max_sweeps = o->svd_max_sweeps;
+    r.svd_tol = o->svd_tol;
   }
   return r;
 }
@@ -78,6 +79,7 @@ void dmrg_default_options(dmrg_options* o) {
   o->profile = d.profile;
   o->device = d.device;
   o->svd_max_sweeps = d.svd_max_sweeps;
+  o->svd_tol = d.svd_tol;
 }
 
 const char* dmrg_last_error(void) { return g_err.c_str(); }
@@ -394,13 +396,28 @@ int dmrg_op_svd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, 
       using T = decltype(tag);
       auto& x = ctx_of<T>(c);
       hipStream_t st = (hipStream_t)stream;
-      x.need_svd(std::max(m, n));
-      SvdResult r = svd_jacobi<T>(st, x.svd, (const T*)M, m, n, m);
-      int nk = svd_rank<T>(st, x.svd, r, k, cutoff, rel_cutoff);
-      svd_write<T>(st, x.svd, r, m, n, nk, scale_left != 0, (T*)left, m, (T*)right, nk);
-      if (sigma) svd_sigma<T>(st, x.svd, r, nk, sigma);
+      x.need_svd(scale_left ? n : m);
+      SvdResult r = svd_split<T>(st, x.svd, (const T*)M, m, n, m, k, cutoff, rel_cutoff,
+                                 scale_left != 0, (T*)left, m, (T*)right, sigma, SvdParams{});
       DMRG_HIP(hipStreamSynchronize(st));
-      *nk_out = nk;
+      *nk_out = r.nk;
+    });
+  });
+}
+
+int dmrg_op_heig(dmrg_ctx_handle c, void* stream, int dtype, int N, const void* H, double* evals,
+                 void* evecs, int* sweeps_out) {
+  if (!c || !H || !evals || !evecs) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
+  return guard([&] {
+    DMRG_REQUIRE(N >= 1 && N <= 2048, "bad shape");
+    by_dtype(dtype, [&](auto tag) {
+      using T = decltype(tag);
+      auto& x = ctx_of<T>(c);
+      hipStream_t st = (hipStream_t)stream;
+      x.need_svd(N);
+      int sw = heig<T>(st, x.svd, (const T*)H, N, N, evals, (T*)evecs, N, SvdParams{});
+      DMRG_HIP(hipStreamSynchronize(st));
+      if (sweeps_out) *sweeps_out = sw;
     });
   });
 }

@@ -400,14 +400,14 @@ void Engine<T>::split_two(int site, char dir) {
   const int m = cL * d, n = d * cR;
   const int full_k = std::min(m, n), k = std::min(full_k, chi);
   double t0 = now_s();
-  SvdResult r = svd_jacobi<T>(s_main, svdw, theta, m, n, m, opt.svd_max_sweeps);
-  int nk = svd_rank<T>(s_main, svdw, r, k, opt.svd_cutoff, opt.svd_rel_cutoff);
-  svd_write<T>(s_main, svdw, r, m, n, nk, dir == 'L', mps[site], m, mps[site + 1], nk);
-  bond[site + 1] = nk;
+  SvdResult r = svd_split<T>(s_main, svdw, theta, m, n, m, k, opt.svd_cutoff, opt.svd_rel_cutoff,
+                             dir == 'L', mps[site], m, mps[site + 1], nullptr, svd_params());
+  bond[site + 1] = r.nk;
   DMRG_HIP(hipEventRecord(ev_canon, s_main));
   stats.svd_ms += 1e3 * (now_s() - t0);
   stats.svd_calls += 1;
   stats.svd_sweeps += r.sweeps;
+  stats.max_discarded = std::max(stats.max_discarded, r.discarded);
 }
 
 template <class T>
@@ -475,10 +475,11 @@ void Engine<T>::split_one(int site, char dir) {
   if (dir == 'R') {
     const int m = cL * d, n = cR;
     const int k = std::min(std::min(m, n), chi);
-    SvdResult r = svd_jacobi<T>(s_main, svdw, theta, m, n, m, opt.svd_max_sweeps);
-    int nk = svd_rank<T>(s_main, svdw, r, k, opt.svd_cutoff, opt.svd_rel_cutoff);
     // A[site] = U; S*Vh (nk x cR) -> work; A[site+1] <- (S Vh) A[site+1]
-    svd_write<T>(s_main, svdw, r, m, n, nk, false, mps[site], m, work, nk);
+    SvdResult r = svd_split<T>(s_main, svdw, theta, m, n, m, k, opt.svd_cutoff,
+                               opt.svd_rel_cutoff, false, mps[site], m, work, nullptr, svd_params());
+    const int nk = r.nk;
+    stats.max_discarded = std::max(stats.max_discarded, r.discarded);
     DMRG_HIP(hipEventRecord(ev_canon, s_main));
     const int cR2 = bond[site + 2];
     gemm<T>(s_main, OP_N, OP_N, nk, d * cR2, cR, T(1), work, nk, mps[site + 1], cR, T(0), mps_spare,
@@ -489,10 +490,11 @@ void Engine<T>::split_one(int site, char dir) {
   } else {
     const int m = cL, n = d * cR;
     const int k = std::min(std::min(m, n), chi);
-    SvdResult r = svd_jacobi<T>(s_main, svdw, theta, m, n, m, opt.svd_max_sweeps);
-    int nk = svd_rank<T>(s_main, svdw, r, k, opt.svd_cutoff, opt.svd_rel_cutoff);
     // A[site] = Vh; U*S (cL x nk) -> work; A[site-1] <- A[site-1] (U S)
-    svd_write<T>(s_main, svdw, r, m, n, nk, true, work, m, mps[site], nk);
+    SvdResult r = svd_split<T>(s_main, svdw, theta, m, n, m, k, opt.svd_cutoff,
+                               opt.svd_rel_cutoff, true, work, m, mps[site], nullptr, svd_params());
+    const int nk = r.nk;
+    stats.max_discarded = std::max(stats.max_discarded, r.discarded);
     DMRG_HIP(hipEventRecord(ev_canon, s_main));
     const int cL0 = bond[site - 1];
     gemm<T>(s_main, OP_N, OP_N, cL0 * d, nk, cL, T(1), mps[site - 1], cL0 * d, work, cL, T(0),

@@ -23,6 +23,7 @@ struct Options {
   int profile = 0;
   int device = 0;
   int svd_max_sweeps = 40;
+  double svd_tol = 1e-14;
 };
 
 struct Stats {
@@ -128,6 +129,12 @@ class Engine {
   ContractScratch<float> sc_f;
   LzDev lz;
   SvdWork<T> svdw;
+  SvdParams svd_params() const {
+    SvdParams p;
+    p.max_sweeps = opt.svd_max_sweeps;
+    p.tol = opt.svd_tol;
+    return p;
+  }
   LanczosRunner<T> runner;
   int* h_status = nullptr;
   double* h_scal = nullptr;

@@ -333,10 +333,10 @@ __device__ double tri_lowest(const double* a, const double* b, int k, double* b2
 }
 
 // Eigenvector of T for eigenvalue lam by inverse iteration (thread 0).
-__device__ void tri_vector(const double* a, const double* b, int k, double lam, double* y) {
+// Work arrays (5 x 128 doubles + 128 ints) live in LDS, passed by the caller.
+__device__ void tri_vector(const double* a, const double* b, int k, double lam, double* y,
+                           double* d, double* du, double* dl, double* du2, double* x, int* piv) {
   if (k == 1) { y[0] = 1.0; return; }
-  double d[128], du[128], dl[128], du2[128], x[128];
-  int piv[128];
   double tnorm = 0.0;
   for (int i = 0; i < k; ++i)
     tnorm = fmax(tnorm, fabs(a[i]) + (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0));
@@ -403,6 +403,8 @@ __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* b
   __shared__ double b2[128];
   __shared__ int cnt[64];
   __shared__ int bidx;
+  __shared__ double wd[128], wdu[128], wdl[128], wdu2[128], wx[128];
+  __shared__ int wpiv[128];
   const int t = threadIdx.x;
   if (mode == 0) {  // check
     if (t == 0) {
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* b
     double lam = (k == 1) ? alpha[0] : tri_lowest(alpha, beta, k, b2, cnt);
     if (t == 0) {
       scal[3] = lam;
-      tri_vector(alpha, beta, k, lam, y);
+      tri_vector(alpha, beta, k, lam, y, wd, wdu, wdl, wdu2, wx, wpiv);
     }
   }
 }

@@ -1,253 +1,115 @@
-// Block one-sided Jacobi SVD (see svd.hpp).
+// Density-matrix truncated SVD and block-Jacobi Hermitian eigen-solver
+// (see svd.hpp).
 #include "svd.hpp"
+#include "gemm.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <algorithm>
-#include <vector>
 
 namespace dmrg {
 namespace {
 
-constexpr int BW = 8;          // column block width
-constexpr int PW = 2 * BW;     // columns per block pair
-constexpr int ROWS = 256;      // rows per slice
-constexpr int INNER_SWEEPS = 3;   // inner sweeps per block pair and round
+constexpr int JB = 16;       // Jacobi block width
+constexpr int JP = 2 * JB;   // order of one pair sub-problem
+constexpr int LDP = JP + 1;  // LDS column stride (padding)
+constexpr int NT = 256;      // threads of the pair kernels
+constexpr int SORT_MAX = 2048;
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
 
 // Round-robin tournament: block pair k of round r among nb blocks (nb even).
 __device__ __host__ inline void rr_pair(int nb, int r, int k, int& a, int& b) {
-  int n1 = nb - 1;
+  const int n1 = nb - 1;
   if (k == 0) { a = r; b = n1; }
   else { a = (r + k) % n1; b = (r - k + n1) % n1; }
 }
-__device__ inline int pcol(int bi, int bj, int c) { return c < BW ? bi * BW + c : bj * BW + (c - BW); }
+// Global row/column of local index i (0..JP-1) of the pair (I, J).
+__device__ inline int gidx(int I, int J, int i) { return i < JB ? I * JB + i : J * JB + (i - JB); }
+// Inner circle schedule over the JP local indices: round rd pairs positions
+// (q, JP-1 - q); position 0 is fixed and the others rotate.
+__device__ inline int cpos(int pos, int rd) { return pos == 0 ? 0 : 1 + (pos - 1 + rd) % (JP - 1); }
 
 __device__ inline double absv(double a) { return fabs(a); }
 __device__ inline double absv(cdouble a) { return sqrt(a.x * a.x + a.y * a.y); }
-__device__ inline double unit_phase(double g, double) { return g >= 0.0 ? 1.0 : -1.0; }
-__device__ inline cdouble unit_phase(cdouble g, double ag) { return cdouble(g.x / ag, g.y / ag); }
 
-// W = A (m >= n) or A^H (m < n), zero-padded to qp columns; V = I.
+template <class T> struct NP { static constexpr int P = 1; };
+template <> struct NP<cdouble> { static constexpr int P = 2; };
+
+__device__ inline void put(double* pl, int idx, int plane_stride, double v) { pl[idx] = v; }
+__device__ inline void put(double* pl, int idx, int plane_stride, cdouble v) {
+  pl[idx] = v.x;
+  pl[idx + plane_stride] = v.y;
+}
+template <class T> __device__ inline T get(const double* pl, int idx, int plane_stride);
+template <> __device__ inline double get<double>(const double* pl, int idx, int) { return pl[idx]; }
+template <> __device__ inline cdouble get<cdouble>(const double* pl, int idx, int ps) {
+  return cdouble(pl[idx], pl[idx + ps]);
+}
+
+// ---------------------------------------------------------------------------
+// Set-up: symmetrise the N x N block of rho (lower := conj(upper), real
+// diagonal), zero the padding up to Np, V = I.
 template <class T>
-__global__ void k_init(const T* __restrict__ A, int m, int n, int lda, int trans, T* __restrict__ W,
-                       int p, int q, int qp, T* __restrict__ V) {
-  size_t totW = (size_t)p * qp, totV = (size_t)qp * qp;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < totW + totV;
-       i += (size_t)gridDim.x * blockDim.x) {
-    if (i < totW) {
-      int r = (int)(i % p), c = (int)(i / p);
-      T v = T(0);
-      if (c < q) v = trans ? conj(A[c + (size_t)r * lda]) : A[r + (size_t)c * lda];
-      W[i] = v;
-    } else {
-      size_t j = i - totW;
-      int r = (int)(j % qp), c = (int)(j / qp);
-      V[j] = (r == c) ? T(1) : T(0);
-    }
+__global__ void k_prep(T* __restrict__ A, T* __restrict__ V, int N, int Np) {
+  const size_t tot = (size_t)Np * Np;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % Np), c = (int)(e / Np);
+    if (r >= N || c >= N) A[e] = T(0);
+    else if (r > c) A[e] = conj(A[c + (size_t)r * Np]);
+    else if (r == c) A[e] = T(real(A[e]));
+    V[e] = (r == c) ? T(1) : T(0);
   }
 }
 
-// Partial 16x16 Gram of a block pair over one row slice: G[u][v] = w_u^H w_v.
+// scal[0] = |rho|_F, scal[2] = trace(rho) (one workgroup).
 template <class T>
-__global__ __launch_bounds__(256) void k_gram(const T* __restrict__ W, int p, int nb, int round,
-                                              int nsplit, int rows, T* __restrict__ gram) {
-  const int k = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
-  int bi, bj;
-  rr_pair(nb, round, k, bi, bj);
-  __shared__ T tile[64][PW + 1];
-  const int u = t % PW, v = t / PW;
-  const int r0 = s * rows, r1 = min(p, r0 + rows);
-  T acc = T(0);
-  for (int rb = r0; rb < r1; rb += 64) {
-    for (int e = t; e < 64 * PW; e += 256) {
-      int rr = e % 64, c = e / 64, gr = rb + rr;
-      tile[rr][c] = (gr < r1) ? W[gr + (size_t)pcol(bi, bj, c) * p] : T(0);
-    }
-    __syncthreads();
-    const int nr = min(64, r1 - rb);
-    for (int rr = 0; rr < nr; ++rr) acc += conj(tile[rr][u]) * tile[rr][v];
-    __syncthreads();
-  }
-  gram[((size_t)k * nsplit + s) * (PW * PW) + u * PW + v] = acc;
-}
-
-struct Rot {
-  double c, s;
-  int p, q;
-};
-
-// Sum the Gram slices of a pair, diagonalise it by cyclic Jacobi (one wave),
-// and apply the accumulated unitary to this slice's rows of W and V.
-template <class T>
-__global__ __launch_bounds__(64) void k_rotate(T* __restrict__ W, int p, T* __restrict__ V, int qp,
-                                               int nb, int round, int nsplit, int rows,
-                                               const T* __restrict__ gram, double tol,
-                                               unsigned long long* __restrict__ off_max) {
-  const int k = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
-  int bi, bj;
-  rr_pair(nb, round, k, bi, bj);
-  __shared__ T G[PW][PW + 1];
-  __shared__ T Q[PW][PW + 1];
-  __shared__ Rot rot[BW];
-  __shared__ T ph[BW];
-  __shared__ double red[64];
-  __shared__ int any;
-
-  for (int e = t; e < PW * PW; e += 64) {
-    int u = e / PW, v = e % PW;
-    T acc = T(0);
-    for (int j = 0; j < nsplit; ++j) acc += gram[((size_t)k * nsplit + j) * (PW * PW) + u * PW + v];
-    G[u][v] = acc;
-    Q[u][v] = (u == v) ? T(1) : T(0);
-  }
-  __syncthreads();
-  // Hermitian symmetrise: keep the upper triangle, real diagonal.
-  for (int e = t; e < PW * PW; e += 64) {
-    int u = e / PW, v = e % PW;
-    if (u > v) G[u][v] = conj(G[v][u]);
-    if (u == v) G[u][u] = T(real(G[u][u]));
-  }
-  __syncthreads();
-  // Relative off-diagonal measure of this pair.
-  double mx = 0.0;
-  for (int e = t; e < PW * PW; e += 64) {
-    int u = e / PW, v = e % PW;
-    if (u < v) {
-      double a = real(G[u][u]), b = real(G[v][v]);
-      if (a > 0.0 && b > 0.0) mx = fmax(mx, absv(G[u][v]) / sqrt(a * b));
-    }
-  }
-  red[t] = mx;
-  __syncthreads();
-  for (int w = 32; w > 0; w >>= 1) {
-    if (t < w) red[t] = fmax(red[t], red[t + w]);
-    __syncthreads();
-  }
-  const double off = red[0];
-  if (s == 0 && t == 0) atomicMax(off_max, (unsigned long long)__double_as_longlong(off));
-  if (!(off > tol)) return;  // already orthogonal (NaN falls through to rotate)
-
-  // Cyclic Jacobi on the 16x16 Hermitian G, accumulating Q (G <- Q^H G Q).
-  // A round rotates 8 disjoint index pairs; the two-sided update then splits
-  // into 64 independent 2x2 blocks G[(pa,qa),(pb,qb)] <- Ja^H G Jb (one per
-  // lane) plus the column rotation of Q, so a round needs two barriers.
-  const double tin = tol * 0.25;
-  for (int sw = 0; sw < INNER_SWEEPS; ++sw) {
-    if (t == 0) any = 0;
-    __syncthreads();
-    for (int r = 0; r < PW - 1; ++r) {
-      if (t < BW) {
-        int a, b;
-        rr_pair(PW, r, t, a, b);
-        double al = real(G[a][a]), be = real(G[b][b]);
-        T g = G[a][b];
-        double ag = absv(g);
-        Rot R{1.0, 0.0, a, b};
-        T e = T(1);
-        if (al > 0.0 && be > 0.0 && ag > tin * sqrt(al * be)) {
-          double zeta = (be - al) / (2.0 * ag);
-          double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          double c = 1.0 / sqrt(1.0 + tt * tt);
-          R.c = c;
-          R.s = c * tt;
-          e = unit_phase(g, ag);
-          any = 1;
-        }
-        rot[t] = R;
-        ph[t] = e;
-      }
-      __syncthreads();
-      {
-        // 2x2 block (ka, kb) of G: columns by Jb, then rows by Ja^H, where
-        // J = [[c, s], [-s conj(e), c conj(e)]] on the (p, q) pair.
-        const int ka = t / BW, kb = t % BW;
-        const Rot Ra = rot[ka], Rb = rot[kb];
-        if (Ra.s != 0.0 || Rb.s != 0.0) {
-          const T ea = ph[ka], eb = conj(ph[kb]);
-          T g00 = G[Ra.p][Rb.p], g01 = G[Ra.p][Rb.q], g10 = G[Ra.q][Rb.p], g11 = G[Ra.q][Rb.q];
-          T x00 = Rb.c * g00 - Rb.s * (eb * g01), x01 = Rb.s * g00 + Rb.c * (eb * g01);
-          T x10 = Rb.c * g10 - Rb.s * (eb * g11), x11 = Rb.s * g10 + Rb.c * (eb * g11);
-          G[Ra.p][Rb.p] = Ra.c * x00 - Ra.s * (ea * x10);
-          G[Ra.p][Rb.q] = Ra.c * x01 - Ra.s * (ea * x11);
-          G[Ra.q][Rb.p] = Ra.s * x00 + Ra.c * (ea * x10);
-          G[Ra.q][Rb.q] = Ra.s * x01 + Ra.c * (ea * x11);
-        }
-        // Q <- Q J (column rotations), 128 (row, pair) tasks over 64 lanes.
-        for (int task = t; task < PW * BW; task += 64) {
-          const int i = task / BW, kk = task % BW;
-          const Rot R = rot[kk];
-          if (R.s == 0.0) continue;
-          const T ce = conj(ph[kk]);
-          T xa = Q[i][R.p], xb = Q[i][R.q];
-          Q[i][R.p] = R.c * xa - R.s * (ce * xb);
-          Q[i][R.q] = R.s * xa + R.c * (ce * xb);
-        }
-      }
-      __syncthreads();
-    }
-    const int more = any;
-    __syncthreads();
-    if (!more) break;
-  }
-
-  // Apply Q to the pair's columns of W (rows of this slice) and of V.
-  const int r0 = s * rows;
-  for (int pass = 0; pass < 2; ++pass) {
-    T* M = pass ? V : W;
-    const int ld = pass ? qp : p;
-    const int r1 = min(ld, r0 + rows);
-    for (int r = r0 + t; r < r1; r += 64) {
-      T x[PW];
-#pragma unroll
-      for (int c = 0; c < PW; ++c) x[c] = M[r + (size_t)pcol(bi, bj, c) * ld];
-#pragma unroll
-      for (int c = 0; c < PW; ++c) {
-        T acc = T(0);
-#pragma unroll
-        for (int u = 0; u < PW; ++u) acc += x[u] * Q[u][c];
-        M[r + (size_t)pcol(bi, bj, c) * ld] = acc;
-      }
-    }
-  }
-}
-
-template <class T>
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
-
-// sigma_c = |w_c| for c < q.
-template <class T>
-__global__ __launch_bounds__(256) void k_colnorm(const T* __restrict__ W, int p, double* sigma) {
-  const int c = blockIdx.x, t = threadIdx.x;
-  __shared__ double red[256];
-  double acc = 0.0;
-  for (int r = t; r < p; r += 256) acc += abs2(W[r + (size_t)c * p]);
-  red[t] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
-  }
-  if (t == 0) sigma[c] = sqrt(red[0]);
-}
-
-// Stable descending sort of q <= 2048 values (bitonic on (sigma, index)).
-__global__ __launch_bounds__(1024) void k_sort(const double* sigma, int q, int* perm) {
-  __shared__ double key[2048];
-  __shared__ int idx[2048];
+__global__ __launch_bounds__(1024) void k_norms(const T* __restrict__ A, int N, int Np,
+                                                double* scal) {
+  __shared__ double s1[1024], s2[1024];
   const int t = threadIdx.x;
-  for (int i = t; i < 2048; i += 1024) {
-    key[i] = (i < q) ? sigma[i] : -1.0;
+  double f = 0.0, tr = 0.0;
+  for (size_t e = t; e < (size_t)Np * Np; e += 1024) f += abs2(A[e]);
+  for (int i = t; i < N; i += 1024) tr += real(A[i + (size_t)i * Np]);
+  s1[t] = f;
+  s2[t] = tr;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o) { s1[t] += s1[t + o]; s2[t] += s2[t + o]; }
+    __syncthreads();
+  }
+  if (t == 0) { scal[0] = sqrt(s1[0]); scal[2] = s2[0]; }
+}
+
+// Stable descending sort of n <= 2048 keys -> perm (and the sorted keys).
+// Key i is the real diagonal A(i, i) (ld Np) when A is given, else keys[i];
+// positions i >= n_real (zero padding of the Jacobi matrix) get -inf, so the
+// padding always sorts to the end and real indices keep positions < n_real.
+template <class T>
+__global__ __launch_bounds__(1024) void k_sort(const T* __restrict__ A, int Np,
+                                               const double* __restrict__ keys, int n, int n_real,
+                                               int* __restrict__ perm, double* __restrict__ keys_out) {
+  __shared__ double key[SORT_MAX];
+  __shared__ int idx[SORT_MAX];
+  const int t = threadIdx.x;
+  int sz = 64;  // bitonic network size: the next power of two >= n
+  while (sz < n) sz <<= 1;
+  for (int i = t; i < sz; i += 1024) {
+    double v = -INFINITY;
+    if (i < n_real) v = A ? real(A[i + (size_t)i * Np]) : keys[i];
+    if (!(v == v)) v = -INFINITY;  // keeps every output index below n
+    key[i] = v;
     idx[i] = i;
   }
   __syncthreads();
-  for (int size = 2; size <= 2048; size <<= 1) {
+  for (int size = 2; size <= sz; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = t; i < 2048; i += 1024) {
-        int j = i ^ stride;
+      for (int i = t; i < sz; i += 1024) {
+        const int j = i ^ stride;
         if (j > i) {
-          bool desc = ((i & size) == 0);
-          // "a before b" when larger key, ties broken by smaller index.
-          bool ib = (key[i] > key[j]) || (key[i] == key[j] && idx[i] < idx[j]);
+          const bool desc = ((i & size) == 0);
+          const bool ib = (key[i] > key[j]) || (key[i] == key[j] && idx[i] < idx[j]);
           if (desc ? !ib : ib) {
             double tk = key[i]; key[i] = key[j]; key[j] = tk;
             int ti = idx[i]; idx[i] = idx[j]; idx[j] = ti;
@@ -257,212 +119,635 @@ __global__ __launch_bounds__(1024) void k_sort(const double* sigma, int q, int* 
       __syncthreads();
     }
   }
-  for (int i = t; i < q; i += 1024) perm[i] = idx[i];
+  for (int i = t; i < n; i += 1024) {
+    perm[i] = idx[i];
+    if (keys_out) keys_out[i] = key[i];
+  }
 }
 
-__global__ void k_rank(const double* sigma, const int* perm, int k, double cutoff, double rel,
-                       int* newk) {
-  double thr = fmax(cutoff, rel * sigma[perm[0]]);
+// B(i, j) = A(p_i, p_j); W(:, j) = V(:, p_j).
+template <class T>
+__global__ void k_permute(const T* __restrict__ A, const T* __restrict__ V, T* __restrict__ B,
+                          T* __restrict__ W, const int* __restrict__ perm, int Np) {
+  const size_t tot = (size_t)Np * Np;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % Np), c = (int)(e / Np);
+    const int pc = perm[c];
+    B[e] = A[perm[r] + (size_t)pc * Np];
+    W[e] = V[r + (size_t)pc * Np];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel A: one workgroup per block pair of round `round`.  Loads the JP x JP
+// Hermitian sub-matrix, runs up to `inner_max` cyclic Jacobi sweeps on it in
+// LDS (skipping pairs with |a_ab| <= rot_tol * |rho|_F), writes the rotated
+// sub-matrix back and the accumulated unitary to Qout (JP x JP, ld JP).
+//
+// Rotation of the pair (a, b): with beta = a_ab = |beta| e^{i phi},
+// tau = (a_bb - a_aa) / (2 |beta|), t = sign(tau) / (|tau| + sqrt(1 + tau^2)),
+// c = 1/sqrt(1+t^2), s = t c, J = [[c, s e^{i phi}], [-s e^{-i phi}, c]];
+// J^H [[a_aa, beta], [conj(beta), a_bb]] J = diag(a_aa - t|beta|, a_bb + t|beta|)
+// (the symmetric Schur decomposition with a phase, Golub & Van Loan 8.4).
+template <class T>
+__global__ __launch_bounds__(NT) void k_pair_eig(T* __restrict__ A, int ld, int nb, int round,
+                                                 T* __restrict__ Qout, int* __restrict__ qflag,
+                                                 const double* __restrict__ scal, double rot_tol,
+                                                 int inner_max) {
+  __shared__ T As[JP * LDP];  // As[c * LDP + r] = sub(r, c)
+  __shared__ T Qs[JP * LDP];
+  __shared__ double rc[JP / 2];
+  __shared__ T rs[JP / 2];
+  __shared__ int rf[JP / 2];
+  __shared__ int s_rot, s_any;
+  const int k = blockIdx.x, t = threadIdx.x;
+  int I, J;
+  rr_pair(nb, round, k, I, J);
+  const double thr = rot_tol * scal[0];
+  const double thr2 = thr * thr;
+  int big = 0;
+  for (int e = t; e < JP * JP; e += NT) {
+    const int r = e % JP, c = e / JP;
+    const T v = A[(size_t)gidx(I, J, r) + (size_t)gidx(I, J, c) * ld];
+    As[c * LDP + r] = v;
+    Qs[c * LDP + r] = (r == c) ? T(1) : T(0);
+    if (r != c && absv(v) > thr) big = 1;
+  }
+  if (t == 0) { s_rot = 0; s_any = 0; }
+  if (!__syncthreads_or(big)) {
+    if (t == 0) qflag[k] = 0;
+    return;
+  }
+  // Circle-schedule indices, advanced incrementally each round (index 0 is
+  // fixed; the others step 1 -> 2 -> ... -> JP-1 -> 1), starting from
+  // position == index.  After JP-1 rounds they are back at the start.
+  auto adv = [](int i) { return i == 0 ? 0 : (i == JP - 1 ? 1 : i + 1); };
+  // HP = JP/2 pairs per round.  Thread t updates the 2x2 blocks of row pair
+  // k1 = t / TPR and column pairs k2 = t % TPR + TPR*q (TPR = NT/HP threads
+  // per row pair), and the Q entries of row t % JP in pairs
+  // kk = t / JP + (NT/JP)*q.
+  constexpr int HP = JP / 2;
+  constexpr int TPR = NT / HP;
+  constexpr int NBQ = HP / TPR, NQQ = JP * HP / NT;
+  static_assert(NT % HP == 0 && HP % TPR == 0 && NT % JP == 0 && (JP * HP) % NT == 0, "mapping");
+  const int k1 = t / TPR;
+  int a1 = k1, b1 = JP - 1 - k1;
+  int a2[NBQ], b2[NBQ], aq[NQQ], bq[NQQ];
+#pragma unroll
+  for (int q = 0; q < NBQ; ++q) { a2[q] = t % TPR + TPR * q; b2[q] = JP - 1 - a2[q]; }
+#pragma unroll
+  for (int q = 0; q < NQQ; ++q) { aq[q] = t / JP + (NT / JP) * q; bq[q] = JP - 1 - aq[q]; }
+  int ap = t % HP, bp = JP - 1 - (t % HP);
+  const int row = t % JP;
+  for (int sw = 0; sw < inner_max; ++sw) {
+    for (int rd = 0; rd < JP - 1; ++rd) {
+      if (t < JP / 2) {
+        const int a = ap, b = bp;
+        const double al = real(As[a * LDP + a]), ga = real(As[b * LDP + b]);
+        const T be = As[b * LDP + a];
+        const double mag2 = abs2(be);
+        double c = 1.0;
+        T sp = T(0);
+        int f = 0;
+        if (mag2 > thr2) {
+          // t = sign(tau)/(|tau| + sqrt(1+tau^2)) with tau = dl/(2|beta|), written
+          // as t = 2|beta| sgn w, w = 1/(|dl| + sqrt(dl^2 + 4|beta|^2)), so one
+          // sqrt, one reciprocal and one rsqrt give c, s e^{i phi} and t|beta|.
+          const double dl = ga - al;
+          const double sg = dl >= 0.0 ? 1.0 : -1.0;
+          const double w = 1.0 / (fabs(dl) + sqrt(dl * dl + 4.0 * mag2));
+          const double tb = 2.0 * sg * mag2 * w;   // t |beta|
+          c = rsqrt(1.0 + 4.0 * mag2 * w * w);     // 1/sqrt(1+t^2)
+          sp = be * (2.0 * sg * c * w);            // s e^{i phi} = t c beta/|beta|
+          As[a * LDP + a] = T(al - tb);
+          As[b * LDP + b] = T(ga + tb);
+          As[b * LDP + a] = T(0);
+          As[a * LDP + b] = T(0);
+          f = 1;
+          s_rot = 1;
+        }
+        rc[t] = c;
+        rs[t] = sp;
+        rf[t] = f;
+      }
+      __syncthreads();
+      // Off-diagonal 2x2 blocks: X <- J1^H X J2.
+      const int f1 = rf[k1];
+      const double c1 = rc[k1];
+      const T s1 = rs[k1];
+#pragma unroll
+      for (int q = 0; q < NBQ; ++q) {
+        const int k2 = t % TPR + TPR * q;
+        if (k1 == k2 || !(f1 | rf[k2])) continue;
+        const T x00 = As[a2[q] * LDP + a1], x01 = As[b2[q] * LDP + a1];
+        const T x10 = As[a2[q] * LDP + b1], x11 = As[b2[q] * LDP + b1];
+        const double c2 = rc[k2];
+        const T s2 = rs[k2];
+        const T y00 = c2 * x00 - conj(s2) * x01, y01 = s2 * x00 + c2 * x01;
+        const T y10 = c2 * x10 - conj(s2) * x11, y11 = s2 * x10 + c2 * x11;
+        As[a2[q] * LDP + a1] = c1 * y00 - s1 * y10;
+        As[b2[q] * LDP + a1] = c1 * y01 - s1 * y11;
+        As[a2[q] * LDP + b1] = conj(s1) * y00 + c1 * y10;
+        As[b2[q] * LDP + b1] = conj(s1) * y01 + c1 * y11;
+      }
+      // Q <- Q J.
+#pragma unroll
+      for (int q = 0; q < NQQ; ++q) {
+        const int kk = t / JP + (NT / JP) * q;
+        if (!rf[kk]) continue;
+        const T qa = Qs[aq[q] * LDP + row], qb = Qs[bq[q] * LDP + row];
+        const double c = rc[kk];
+        const T s = rs[kk];
+        Qs[aq[q] * LDP + row] = c * qa - conj(s) * qb;
+        Qs[bq[q] * LDP + row] = s * qa + c * qb;
+      }
+      a1 = adv(a1);
+      b1 = adv(b1);
+#pragma unroll
+      for (int q = 0; q < NBQ; ++q) { a2[q] = adv(a2[q]); b2[q] = adv(b2[q]); }
+#pragma unroll
+      for (int q = 0; q < NQQ; ++q) { aq[q] = adv(aq[q]); bq[q] = adv(bq[q]); }
+      ap = adv(ap);
+      bp = adv(bp);
+      __syncthreads();
+    }
+    const int rot = s_rot;
+    __syncthreads();
+    if (t == 0) { s_any |= rot; s_rot = 0; }
+    __syncthreads();
+    if (!rot) break;
+  }
+  const int any = s_any;
+  if (t == 0) qflag[k] = any;
+  if (!any) return;
+  for (int e = t; e < JP * JP; e += NT) {
+    const int r = e % JP, c = e / JP;
+    A[(size_t)gidx(I, J, r) + (size_t)gidx(I, J, c) * ld] = As[c * LDP + r];
+    Qout[(size_t)k * JP * JP + e] = Qs[c * LDP + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel B: apply the round's block-diagonal unitary P (pair blocks Q_P):
+//   off-diagonal pair blocks  rho_PQ <- Q_P^H rho_PQ Q_Q  (and the mirror),
+//   eigenvector tiles         V(rows, P) <- V(rows, P) Q_P.
+// The diagonal pair blocks were written by k_pair_eig.  JP x JP x JP products
+// on v_mfma_f64_16x16x4_f64 (complex: four real MFMAs on re/im planes); wave w
+// owns the 16x16 output tiles w, w+4, ...  The MFMA is issued with the right operand
+// in the first slot, so lane l's accumulator register r holds output
+// (row l%16, column l/16 + 4r) of its 16x16 tile.
+constexpr int TD = JP / 16;        // 16x16 output tiles per dimension
+constexpr int TPW = TD * TD / 4;   // tiles per wave (4 waves)
+static_assert(TPW >= 1 && (TD * TD) % 4 == 0, "tile mapping");
+
+template <class T>
+__device__ inline void tile_mm(const double* __restrict__ Ls, bool ltrans, const double* __restrict__ Rs,
+                               d4_t (&acc)[NP<T>::P][TPW], int wave, int lane) {
+  constexpr int P = NP<T>::P;
+  constexpr int PS = JP * LDP;
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) acc[p][u] = d4_t{0.0, 0.0, 0.0, 0.0};
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < JP; kk += 4) {
+    const int kx = kk + lk;
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int id = wave + 4 * u, it = id % TD, jt = id / TD;
+      const int i = it * 16 + li, j = jt * 16 + li;
+      // right operand R(kx, j) stored R[j * LDP + kx]
+      const double br = Rs[j * LDP + kx];
+      // left operand L(i, kx) = X[kx * LDP + i], or conj(X[i * LDP + kx]) when transposed
+      const int ia = ltrans ? (i * LDP + kx) : (kx * LDP + i);
+      const double ar = Ls[ia];
+      acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(br, ar, acc[0][u], 0, 0, 0);
+      if (P == 2) {
+        const double bi = Rs[PS + j * LDP + kx];
+        double ai = Ls[PS + ia];
+        if (ltrans) ai = -ai;
+        // re += ar*br - ai*bi ; im += ar*bi + ai*br
+        acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bi, ai, acc[0][u], 0, 0, 0);
+        acc[P - 1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi, ar, acc[P - 1][u], 0, 0, 0);
+        acc[P - 1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(br, ai, acc[P - 1][u], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// Write accumulators (tile layout above) into LDS planes Xs (col-major, LDP).
+template <class T>
+__device__ inline void acc_to_lds(double* __restrict__ Xs, d4_t (&acc)[NP<T>::P][TPW], int wave,
+                                  int lane) {
+  constexpr int P = NP<T>::P;
+  constexpr int PS = JP * LDP;
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    const int id = wave + 4 * u, it = id % TD, jt = id / TD;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = it * 16 + li, j = jt * 16 + lk + 4 * r;
+#pragma unroll
+      for (int p = 0; p < P; ++p) Xs[p * PS + j * LDP + i] = acc[p][u][r];
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_pair_apply(T* __restrict__ A, T* __restrict__ V, int ld,
+                                                   int nb, int round, const T* __restrict__ Q,
+                                                   const int* __restrict__ qflag) {
+  constexpr int P = NP<T>::P;
+  constexpr int PS = JP * LDP;
+  __shared__ double Xs[P * PS];
+  __shared__ double Rs[P * PS];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int np = nb / 2;
+  const int noff = np * (np - 1) / 2;
+  int bid = blockIdx.x;
+  if (bid < noff) {
+    // decode the pair block (p1, p2), p1 < p2
+    int p1 = 0;
+    while (bid >= np - 1 - p1) { bid -= np - 1 - p1; ++p1; }
+    const int p2 = p1 + 1 + bid;
+    const int f1 = qflag[p1], f2 = qflag[p2];
+    if (!(f1 | f2)) return;
+    int I1, J1, I2, J2;
+    rr_pair(nb, round, p1, I1, J1);
+    rr_pair(nb, round, p2, I2, J2);
+    for (int e = t; e < JP * JP; e += NT) {
+      const int r = e % JP, c = e / JP;
+      put(Xs, c * LDP + r, PS, A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * ld]);
+      if (f2) put(Rs, c * LDP + r, PS, Q[(size_t)p2 * JP * JP + e]);
+    }
+    __syncthreads();
+    d4_t acc[P][TPW];
+    if (f2) {
+      tile_mm<T>(Xs, false, Rs, acc, wave, lane);  // X Q2
+      __syncthreads();
+      acc_to_lds<T>(Xs, acc, wave, lane);
+    }
+    if (f1) {
+      for (int e = t; e < JP * JP; e += NT) {
+        const int r = e % JP, c = e / JP;
+        put(Rs, c * LDP + r, PS, Q[(size_t)p1 * JP * JP + e]);
+      }
+      __syncthreads();
+      tile_mm<T>(Rs, true, Xs, acc, wave, lane);  // Q1^H (X Q2)
+      __syncthreads();
+      acc_to_lds<T>(Xs, acc, wave, lane);
+    }
+    __syncthreads();
+    for (int e = t; e < JP * JP; e += NT) {
+      const int r = e % JP, c = e / JP;
+      A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * ld] = get<T>(Xs, c * LDP + r, PS);
+      // mirror: rho(g2(r), g1(c)) = conj(Z(c, r))
+      A[(size_t)gidx(I2, J2, r) + (size_t)gidx(I1, J1, c) * ld] = conj(get<T>(Xs, r * LDP + c, PS));
+    }
+  } else {
+    bid -= noff;
+    const int pp = bid % np, rt = bid / np;
+    if (!qflag[pp]) return;
+    int I, J;
+    rr_pair(nb, round, pp, I, J);
+    const size_t r0 = (size_t)rt * JP;
+    for (int e = t; e < JP * JP; e += NT) {
+      const int r = e % JP, c = e / JP;
+      put(Xs, c * LDP + r, PS, V[r0 + r + (size_t)gidx(I, J, c) * ld]);
+      put(Rs, c * LDP + r, PS, Q[(size_t)pp * JP * JP + e]);
+    }
+    __syncthreads();
+    d4_t acc[P][TPW];
+    tile_mm<T>(Xs, false, Rs, acc, wave, lane);
+    __syncthreads();
+    acc_to_lds<T>(Xs, acc, wave, lane);
+    __syncthreads();
+    for (int e = t; e < JP * JP; e += NT) {
+      const int r = e % JP, c = e / JP;
+      V[r0 + r + (size_t)gidx(I, J, c) * ld] = get<T>(Xs, c * LDP + r, PS);
+    }
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// max |A(i,j)|, i != j, over the N x N block -> atomicMax on the bit pattern
+// (non-negative doubles order like their bits; NaN maps to +inf bits + 1).
+template <class T>
+__global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, int Np,
+                                                unsigned long long* out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double mx = 0.0;
+  const size_t tot = (size_t)Np * Np;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + t; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % Np), c = (int)(e / Np);
+    if (r == c || r >= N || c >= N) continue;
+    double v = absv(A[e]);
+    if (!(v == v)) v = INFINITY;
+    mx = fmax(mx, v);
+  }
+  red[t] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmax(red[t], red[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    double v = red[0];
+    unsigned long long bits;
+    memcpy(&bits, &v, sizeof(v));
+    atomicMax(out, bits);
+  }
+}
+
+// out(i, j) = V(i, perm[j]) (conj-transposed into out(j, i) with `herm`).
+template <class T>
+__global__ void k_gather(const T* __restrict__ V, int ldv, const int* __restrict__ perm, int rows,
+                         int cols, int herm, T* __restrict__ out, int ldo) {
+  const size_t tot = (size_t)rows * cols;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    if (!herm) {
+      const int i = (int)(e % rows), j = (int)(e / rows);
+      out[i + (size_t)j * ldo] = V[i + (size_t)perm[j] * ldv];
+    } else {
+      const int j = (int)(e % cols), i = (int)(e / cols);
+      out[j + (size_t)i * ldo] = conj(V[i + (size_t)perm[j] * ldv]);
+    }
+  }
+}
+
+// sigma_j of the QR step.  G = Y^H Y is the Gram matrix of the product
+// factor's k vectors (ordered by eigenvalue); sigma_j^2 is the residual of
+// vector j after projecting out the earlier ones (R_jj^2 of G = R^H R), taken
+// to first order, sigma_j^2 = G_jj - sum_{i<j} |G_ij|^2 / G_ii.  A vector that
+// is numerically parallel to earlier ones -- the image of a null-space
+// eigenvector contaminated by a kept direction -- then gets sigma ~ 0, as the
+// |R_jj| of the QR factorisation of A V in rocsolver_dgesvdj does.
+// Two passes keep the first-order sum from counting a shared direction twice:
+// pass 0 computes every residual with all earlier vectors; pass 1 subtracts
+// only "clean" pivots i (pass-0 residual >= G_ii / 2), so parallel junk
+// vectors never act as pivots.
+template <class T>
+__global__ __launch_bounds__(256) void k_qr_sigma(const T* __restrict__ G, int ldg, int pass,
+                                                  const double* __restrict__ r0,
+                                                  double* __restrict__ out) {
+  __shared__ double red[256];
+  const int j = blockIdx.x, t = threadIdx.x;
+  double s = 0.0;
+  for (int i = t; i < j; i += 256) {
+    const double gii = real(G[i + (size_t)i * ldg]);
+    if (gii > 0.0 && (pass == 0 || r0[i] >= 0.5 * gii)) s += abs2(G[i + (size_t)j * ldg]) / gii;
+  }
+  red[t] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double r = real(G[j + (size_t)j * ldg]) - red[0];
+    out[j] = pass == 0 ? r : sqrt(fmax(r, 0.0));
+  }
+}
+
+// Truncation rank (scalar_traits.h:497-505 semantics, plus the optional
+// relative cutoff) and discarded weight trace - sum_{j<nk} sigma_j^2.
+__global__ void k_rank(const double* __restrict__ sigma, int k, double cutoff, double rel,
+                       double* scal, int* newk) {
+  const double thr = fmax(cutoff, rel * sigma[0]);
   int nk = k;
   for (int i = 0; i < k; ++i) {
-    double sv = sigma[perm[i]];
-    if (sv < thr || sv != sv) { nk = i; break; }
+    const double s = sigma[i];
+    if (s < thr || s != s) { nk = i; break; }
   }
   if (nk == 0) nk = 1;
+  double kept = 0.0;
+  for (int i = 0; i < nk; ++i) kept += sigma[i] * sigma[i];
+  scal[1] = fmax(scal[2] - kept, 0.0);
   *newk = nk;
 }
 
-// left (m x nk): U_x[i, j] (* sigma_j if scale_left)
+// out(j, c) = Y(j, c) for j < nk (ld nk) from Y (ld ldy).
 template <class T>
-__global__ void k_write_left(const T* __restrict__ W, int p, const T* __restrict__ V, int qp,
-                             const double* __restrict__ sigma, const int* __restrict__ perm,
-                             int trans, int m, int nk, int scale_left, T* __restrict__ out,
-                             int ldo) {
-  size_t tot = (size_t)m * nk;
+__global__ void k_copy_rows(const T* __restrict__ Y, int ldy, int nk, int n, T* __restrict__ out) {
+  const size_t tot = (size_t)nk * n;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    int i = (int)(e % m), j = (int)(e / m);
-    int c = perm[j];
-    double sg = sigma[c];
-    T u;
-    if (!trans) u = (sg > 0.0) ? W[i + (size_t)c * p] * (1.0 / sg) : T(0);
-    else u = V[i + (size_t)c * qp];
-    if (scale_left) u = u * sg;
-    out[i + (size_t)j * ldo] = u;
+    const int j = (int)(e % nk), c = (int)(e / nk);
+    out[e] = Y[j + (size_t)c * ldy];
   }
-}
-
-// right (nk x n): conj(V_x[col, j]) (* sigma_j unless scale_left)
-template <class T>
-__global__ void k_write_right(const T* __restrict__ W, int p, const T* __restrict__ V, int qp,
-                              const double* __restrict__ sigma, const int* __restrict__ perm,
-                              int trans, int n, int nk, int scale_left, T* __restrict__ out,
-                              int ldo) {
-  size_t tot = (size_t)nk * n;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
-       e += (size_t)gridDim.x * blockDim.x) {
-    int j = (int)(e % nk), col = (int)(e / nk);
-    int c = perm[j];
-    double sg = sigma[c];
-    T v;
-    if (!trans) v = V[col + (size_t)c * qp];
-    else v = (sg > 0.0) ? W[col + (size_t)c * p] * (1.0 / sg) : T(0);
-    v = conj(v);
-    if (!scale_left) v = v * sg;
-    out[j + (size_t)col * ldo] = v;
-  }
-}
-
-__global__ void k_gather_sigma(const double* sigma, const int* perm, int count, double* out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) out[i] = sigma[perm[i]];
 }
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+inline unsigned grid_for(size_t tot) { return (unsigned)std::min<size_t>(cdiv(tot, 256), 8192); }
 
-}  // namespace
-
+// Block-Jacobi eigen-decomposition of the Hermitian matrix already stored in
+// w.A[0] (N x N, ld Np, padding arbitrary).  Leaves the diagonalised matrix in
+// w.A[cur], the eigenvectors in w.V[cur] and the descending order in w.perm.
 template <class T>
-void SvdWork<T>::alloc(int cap_rows) {
-  release();
-  cap = cap_rows;
-  capq = round_up(cap_rows, PW);
-  int npairs = capq / PW;
-  int nsplit = (capq + ROWS - 1) / ROWS;
-  gram_elems = (size_t)npairs * nsplit * PW * PW;
-  DMRG_HIP(hipMalloc(&W, (size_t)cap * capq * sizeof(T)));
-  DMRG_HIP(hipMalloc(&V, (size_t)capq * capq * sizeof(T)));
-  DMRG_HIP(hipMalloc(&gram, gram_elems * sizeof(T)));
-  DMRG_HIP(hipMalloc(&sigma, capq * sizeof(double)));
-  DMRG_HIP(hipMalloc(&perm, capq * sizeof(int)));
-  DMRG_HIP(hipMalloc(&off, sizeof(unsigned long long)));
-  DMRG_HIP(hipMalloc(&newk, sizeof(int)));
-}
-
-template <class T>
-void SvdWork<T>::release() {
-  if (W) (void)hipFree(W);
-  if (V) (void)hipFree(V);
-  if (gram) (void)hipFree(gram);
-  if (sigma) (void)hipFree(sigma);
-  if (perm) (void)hipFree(perm);
-  if (off) (void)hipFree(off);
-  if (newk) (void)hipFree(newk);
-  W = V = gram = nullptr;
-  sigma = nullptr;
-  perm = nullptr;
-  off = nullptr;
-  newk = nullptr;
-}
-
-template <class T>
-SvdResult svd_jacobi(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int lda,
-                     int max_sweeps) {
-  DMRG_REQUIRE(m > 0 && n > 0 && lda >= m, "svd: bad shape");
-  DMRG_REQUIRE(std::max(m, n) <= w.cap, "svd: matrix exceeds workspace");
-  DMRG_REQUIRE(std::min(m, n) <= 2048, "svd: more than 2048 singular values");
-  SvdResult res;
-  res.transposed = (m < n);
-  res.rows_p = res.transposed ? n : m;
-  res.cols_q = res.transposed ? m : n;
-  const int p = res.rows_p, q = res.cols_q;
-  const int qp = round_up(q, PW);
-  const int nb = qp / BW;
-  const int npairs = nb / 2;
-  const int nsplit = (std::max(p, qp) + ROWS - 1) / ROWS;
-  DMRG_REQUIRE((size_t)npairs * nsplit * PW * PW <= w.gram_elems, "svd: gram workspace");
-
-  size_t tot = (size_t)p * qp + (size_t)qp * qp;
-  hipLaunchKernelGGL(k_init<T>, dim3(std::min<size_t>(cdiv(tot, 256), 4096)), dim3(256), 0, st, A,
-                     m, n, lda, res.transposed ? 1 : 0, w.W, p, q, qp, w.V);
+int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, int& sweeps,
+           double& off_rel) {
+  const int nb = Np / JB;
+  const int np = nb / 2;
+  const size_t tot = (size_t)Np * Np;
+  hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[0], w.V[0], N, Np);
+  hipLaunchKernelGGL(k_norms<T>, dim3(1), dim3(1024), 0, st, w.A[0], N, Np, w.scal);
   DMRG_LAUNCH_CHECK();
-
-  const double eps = 1.1102230246251565e-16;
-  const double tol = std::max(1e-15, std::sqrt((double)p) * eps);
-  res.sweeps = 0;
-  res.off = 0.0;
+  double hs[3] = {0, 0, 0};
+  DMRG_HIP(hipMemcpyAsync(hs, w.scal, sizeof(hs), hipMemcpyDeviceToHost, st));
+  DMRG_HIP(hipStreamSynchronize(st));
+  const double fro = hs[0];
+  if (!(fro == fro) || std::isinf(fro)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
+  const double rot_tol = 0.125 * prm.tol;
+  const int inner_max = (nb == 2) ? 64 : 2;
+  const int noff = np * (np - 1) / 2, nvt = (Np / JP) * np;
+  int cur = 0;
+  sweeps = 0;
+  off_rel = 0.0;
+  if (fro == 0.0) {
+    hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N,
+                       w.perm, nullptr);
+    DMRG_LAUNCH_CHECK();
+    return cur;
+  }
   bool done = false;
-  for (int sw = 0; sw < max_sweeps; ++sw) {
-    hipLaunchKernelGGL(k_zero_u64<T>, dim3(1), dim3(1), 0, st, w.off);
+  for (int sw = 0; sw < prm.max_sweeps; ++sw) {
+    // sort the diagonal, permute rho and V into the other buffer
+    hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N,
+                       w.perm, nullptr);
+    hipLaunchKernelGGL(k_permute<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[cur], w.V[cur],
+                       w.A[1 - cur], w.V[1 - cur], w.perm, Np);
+    cur = 1 - cur;
     for (int r = 0; r < nb - 1; ++r) {
-      hipLaunchKernelGGL(k_gram<T>, dim3(npairs, nsplit), dim3(256), 0, st, w.W, p, nb, r, nsplit,
-                         ROWS, w.gram);
-      hipLaunchKernelGGL(k_rotate<T>, dim3(npairs, nsplit), dim3(64), 0, st, w.W, p, w.V, qp, nb, r,
-                         nsplit, ROWS, w.gram, tol, w.off);
+      hipLaunchKernelGGL(k_pair_eig<T>, dim3(np), dim3(NT), 0, st, w.A[cur], Np, nb, r, w.Q,
+                         w.qflag, (const double*)w.scal, rot_tol, inner_max);
+      hipLaunchKernelGGL(k_pair_apply<T>, dim3(noff + nvt), dim3(NT), 0, st, w.A[cur], w.V[cur], Np,
+                         nb, r, (const T*)w.Q, (const int*)w.qflag);
     }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
+    hipLaunchKernelGGL(k_offmax<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0, st,
+                       w.A[cur], N, Np, w.off);
     DMRG_LAUNCH_CHECK();
     unsigned long long bits = 0;
     DMRG_HIP(hipMemcpyAsync(&bits, w.off, sizeof(bits), hipMemcpyDeviceToHost, st));
     DMRG_HIP(hipStreamSynchronize(st));
     double off;
     std::memcpy(&off, &bits, sizeof(off));
-    res.sweeps = sw + 1;
-    res.off = off;
-    if (!(off == off)) throw Error(ST_NONFINITE, "svd: non-finite Gram entries");
-    if (off <= tol) { done = true; break; }
+    sweeps = sw + 1;
+    off_rel = off / fro;
+    if (std::isinf(off) || !(off == off)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
+    if (off <= prm.tol * fro) { done = true; break; }
   }
-  if (!done) throw Error(ST_SOLVER_NO_CONV, "svd: Jacobi did not converge");
-  hipLaunchKernelGGL(k_colnorm<T>, dim3(q), dim3(256), 0, st, w.W, p, w.sigma);
-  hipLaunchKernelGGL(k_sort, dim3(1), dim3(1024), 0, st, w.sigma, q, w.perm);
+  if (!done) throw Error(ST_SOLVER_NO_CONV, "svd: Jacobi eigen-solver did not converge");
+  hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N, w.perm,
+                     nullptr);
+  DMRG_LAUNCH_CHECK();
+  return cur;
+}
+
+}  // namespace
+
+template <class T>
+void SvdWork<T>::alloc(int cap_n) {
+  release();
+  cap = round_up(std::max(cap_n, 1), JP);
+  DMRG_REQUIRE(cap <= SORT_MAX, "svd: dimension above 2048");
+  const size_t mat = (size_t)cap * cap * sizeof(T);
+  for (int i = 0; i < 2; ++i) {
+    DMRG_HIP(hipMalloc(&A[i], mat));
+    DMRG_HIP(hipMalloc(&V[i], mat));
+  }
+  DMRG_HIP(hipMalloc(&Y, mat));
+  DMRG_HIP(hipMalloc(&Q, (size_t)(cap / JP) * JP * JP * sizeof(T)));
+  DMRG_HIP(hipMalloc(&qflag, (cap / JP) * sizeof(int)));
+  DMRG_HIP(hipMalloc(&perm, cap * sizeof(int)));
+  DMRG_HIP(hipMalloc(&sigma, cap * sizeof(double)));
+  DMRG_HIP(hipMalloc(&resid, cap * sizeof(double)));
+  DMRG_HIP(hipMalloc(&scal, 4 * sizeof(double)));
+  DMRG_HIP(hipMalloc(&off, sizeof(unsigned long long)));
+  DMRG_HIP(hipMalloc(&newk, sizeof(int)));
+}
+
+template <class T>
+void SvdWork<T>::release() {
+  for (int i = 0; i < 2; ++i) {
+    if (A[i]) (void)hipFree(A[i]);
+    if (V[i]) (void)hipFree(V[i]);
+    A[i] = V[i] = nullptr;
+  }
+  void* ps[] = {Y, Q, qflag, perm, sigma, resid, scal, off, newk};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  Y = nullptr;
+  Q = nullptr;
+  qflag = nullptr;
+  perm = nullptr;
+  sigma = nullptr;
+  resid = nullptr;
+  scal = nullptr;
+  off = nullptr;
+  newk = nullptr;
+  cap = 0;
+}
+
+template <class T>
+SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int lda, int kmax,
+                    double cutoff, double rel_cutoff, bool scale_left, T* left, int ldl, T* right,
+                    double* sigma_out, const SvdParams& prm) {
+  DMRG_REQUIRE(m > 0 && n > 0 && lda >= m && kmax >= 1, "svd: bad shape");
+  DMRG_REQUIRE(left && right && ldl >= m, "svd: bad output");
+  const int N = scale_left ? n : m;
+  const int Np = round_up(N, JP);
+  DMRG_REQUIRE(Np <= w.cap, "svd: matrix exceeds workspace");
+  const int k = std::min(std::min(m, n), kmax);
+  // density matrix on the side of the orthonormal factor
+  if (!scale_left)
+    gemm<T>(st, OP_N, OP_H, m, m, n, T(1), A, lda, A, lda, T(0), w.A[0], Np,
+            gemm_pick_tile(m, m, n, is_cplx<T>::value));
+  else
+    gemm<T>(st, OP_H, OP_N, n, n, m, T(1), A, lda, A, lda, T(0), w.A[0], Np,
+            gemm_pick_tile(n, n, m, is_cplx<T>::value));
+  SvdResult res;
+  res.N = N;
+  const int cur = jacobi<T>(st, w, N, Np, prm, res.sweeps, res.off);
+  if (!scale_left) {
+    // U_k -> left; S Vh = U_k^H theta -> Y (k x n); G = Y Y^H
+    hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)m * k)), dim3(256), 0, st,
+                       (const T*)w.V[cur], Np, (const int*)w.perm, m, k, 0, left, ldl);
+    gemm<T>(st, OP_H, OP_N, k, n, m, T(1), left, ldl, A, lda, T(0), w.Y, k,
+            gemm_pick_tile(k, n, m, is_cplx<T>::value));
+    gemm<T>(st, OP_N, OP_H, k, k, n, T(1), w.Y, k, w.Y, k, T(0), w.A[1 - cur], k,
+            gemm_pick_tile(k, k, n, is_cplx<T>::value));
+  } else {
+    // V_k -> Y (n x k); U S = theta V_k -> left; G = left^H left
+    hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)n * k)), dim3(256), 0, st,
+                       (const T*)w.V[cur], Np, (const int*)w.perm, n, k, 0, w.Y, n);
+    gemm<T>(st, OP_N, OP_N, m, k, n, T(1), A, lda, w.Y, n, T(0), left, ldl,
+            gemm_pick_tile(m, k, n, is_cplx<T>::value));
+    gemm<T>(st, OP_H, OP_N, k, k, m, T(1), left, ldl, left, ldl, T(0), w.A[1 - cur], k,
+            gemm_pick_tile(k, k, m, is_cplx<T>::value));
+  }
+  hipLaunchKernelGGL(k_qr_sigma<T>, dim3(k), dim3(256), 0, st, (const T*)w.A[1 - cur], k, 0,
+                     (const double*)nullptr, w.resid);
+  hipLaunchKernelGGL(k_qr_sigma<T>, dim3(k), dim3(256), 0, st, (const T*)w.A[1 - cur], k, 1,
+                     (const double*)w.resid, w.sigma);
+  hipLaunchKernelGGL(k_rank, dim3(1), dim3(1), 0, st, (const double*)w.sigma, k, cutoff, rel_cutoff,
+                     w.scal, w.newk);
+  DMRG_LAUNCH_CHECK();
+  int nk = 0;
+  double disc = 0.0;
+  DMRG_HIP(hipMemcpyAsync(&nk, w.newk, sizeof(int), hipMemcpyDeviceToHost, st));
+  DMRG_HIP(hipMemcpyAsync(&disc, w.scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  DMRG_HIP(hipStreamSynchronize(st));
+  res.nk = nk;
+  res.discarded = disc;
+  if (!scale_left)
+    hipLaunchKernelGGL(k_copy_rows<T>, dim3(grid_for((size_t)nk * n)), dim3(256), 0, st,
+                       (const T*)w.Y, k, nk, n, right);
+  else
+    hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)n * nk)), dim3(256), 0, st,
+                       (const T*)w.V[cur], Np, (const int*)w.perm, n, nk, 1, right, nk);
+  if (sigma_out)
+    DMRG_HIP(hipMemcpyAsync(sigma_out, w.sigma, nk * sizeof(double), hipMemcpyDeviceToDevice, st));
   DMRG_LAUNCH_CHECK();
   return res;
 }
 
 template <class T>
-int svd_rank(hipStream_t st, SvdWork<T>& w, const SvdResult& r, int k, double cutoff,
-             double rel_cutoff) {
-  DMRG_REQUIRE(k >= 1 && k <= r.cols_q, "svd_rank: bad k");
-  hipLaunchKernelGGL(k_rank, dim3(1), dim3(1), 0, st, w.sigma, w.perm, k, cutoff, rel_cutoff,
-                     w.newk);
+int heig(hipStream_t st, SvdWork<T>& w, const T* H, int N, int ldh, double* evals, T* evecs,
+         int lde, const SvdParams& prm) {
+  DMRG_REQUIRE(N > 0 && ldh >= N && lde >= N, "heig: bad shape");
+  const int Np = round_up(N, JP);
+  DMRG_REQUIRE(Np <= w.cap, "heig: matrix exceeds workspace");
+  DMRG_HIP(hipMemcpy2DAsync(w.A[0], (size_t)Np * sizeof(T), H, (size_t)ldh * sizeof(T),
+                            (size_t)N * sizeof(T), N, hipMemcpyDeviceToDevice, st));
+  int sweeps = 0;
+  double off = 0.0;
+  const int cur = jacobi<T>(st, w, N, Np, prm, sweeps, off);
+  hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, (const T*)w.A[cur], Np, nullptr, N, N,
+                     w.perm, evals);
+  hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)N * N)), dim3(256), 0, st,
+                     (const T*)w.V[cur], Np, (const int*)w.perm, N, N, 0, evecs, lde);
   DMRG_LAUNCH_CHECK();
-  int nk = 0;
-  DMRG_HIP(hipMemcpyAsync(&nk, w.newk, sizeof(int), hipMemcpyDeviceToHost, st));
-  DMRG_HIP(hipStreamSynchronize(st));
-  return nk;
-}
-
-template <class T>
-void svd_write(hipStream_t st, SvdWork<T>& w, const SvdResult& r, int m, int n, int nk,
-               bool scale_left, T* left, int ldl, T* right, int ldr) {
-  const int p = r.rows_p, qp = round_up(r.cols_q, PW);
-  int tr = r.transposed ? 1 : 0;
-  if (left) {
-    size_t tot = (size_t)m * nk;
-    hipLaunchKernelGGL(k_write_left<T>, dim3(std::min<size_t>(cdiv(tot, 256), 8192)), dim3(256), 0,
-                       st, w.W, p, w.V, qp, w.sigma, w.perm, tr, m, nk, scale_left ? 1 : 0, left,
-                       ldl);
-  }
-  if (right) {
-    size_t tot = (size_t)nk * n;
-    hipLaunchKernelGGL(k_write_right<T>, dim3(std::min<size_t>(cdiv(tot, 256), 8192)), dim3(256),
-                       0, st, w.W, p, w.V, qp, w.sigma, w.perm, tr, n, nk, scale_left ? 1 : 0,
-                       right, ldr);
-  }
-  DMRG_LAUNCH_CHECK();
-}
-
-template <class T>
-void svd_sigma(hipStream_t st, SvdWork<T>& w, const SvdResult& r, int count, double* out) {
-  DMRG_REQUIRE(count <= r.cols_q, "svd_sigma: count");
-  hipLaunchKernelGGL(k_gather_sigma, dim3(cdiv(count, 256)), dim3(256), 0, st, w.sigma, w.perm,
-                     count, out);
-  DMRG_LAUNCH_CHECK();
+  return sweeps;
 }
 
 template struct SvdWork<double>;
 template struct SvdWork<cdouble>;
-template SvdResult svd_jacobi<double>(hipStream_t, SvdWork<double>&, const double*, int, int, int,
-                                      int);
-template SvdResult svd_jacobi<cdouble>(hipStream_t, SvdWork<cdouble>&, const cdouble*, int, int,
-                                       int, int);
-template int svd_rank<double>(hipStream_t, SvdWork<double>&, const SvdResult&, int, double, double);
-template int svd_rank<cdouble>(hipStream_t, SvdWork<cdouble>&, const SvdResult&, int, double,
-                               double);
-template void svd_write<double>(hipStream_t, SvdWork<double>&, const SvdResult&, int, int, int,
-                                bool, double*, int, double*, int);
-template void svd_write<cdouble>(hipStream_t, SvdWork<cdouble>&, const SvdResult&, int, int, int,
-                                 bool, cdouble*, int, cdouble*, int);
-template void svd_sigma<double>(hipStream_t, SvdWork<double>&, const SvdResult&, int, double*);
-template void svd_sigma<cdouble>(hipStream_t, SvdWork<cdouble>&, const SvdResult&, int, double*);
+template SvdResult svd_split<double>(hipStream_t, SvdWork<double>&, const double*, int, int, int,
+                                     int, double, double, bool, double*, int, double*, double*,
+                                     const SvdParams&);
+template SvdResult svd_split<cdouble>(hipStream_t, SvdWork<cdouble>&, const cdouble*, int, int,
+                                      int, int, double, double, bool, cdouble*, int, cdouble*,
+                                      double*, const SvdParams&);
+template int heig<double>(hipStream_t, SvdWork<double>&, const double*, int, int, double*, double*,
+                          int, const SvdParams&);
+template int heig<cdouble>(hipStream_t, SvdWork<cdouble>&, const cdouble*, int, int, double*,
+                           cdouble*, int, const SvdParams&);
 
 }  // namespace dmrg

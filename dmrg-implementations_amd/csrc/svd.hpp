@@ -1,16 +1,36 @@
-// FP64 / complex128 SVD for the two-site truncation (replaces the reference's
-// rocsolver_gesvdj / gesvd call, gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h:1193-1202,
-// gpu-rocm/common/scalar_traits.h:178-188 and :375-391).
+// FP64 / complex128 truncated SVD for the two-site (and single-site) split
+// (replaces the reference's rocsolver_gesvdj / gesvd call and its truncation
+// kernels: gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h:1166-1343,
+// gpu-rocm/common/scalar_traits.h:178-188, :375-391 and :497-547).
 //
-// Algorithm: block one-sided Jacobi (Hestenes).  The working matrix W (p x q,
-// q <= p; the input is conjugate-transposed first when it is wide) is split
-// into column blocks of 8.  Each round pairs all blocks (round-robin); for
-// every block pair a 16x16 Gram matrix is accumulated over row slices,
-// diagonalised by cyclic two-sided Jacobi in LDS, and the resulting unitary
-// is applied to the pair's 16 columns of W and of the accumulated V.  Sweeps
-// repeat until every pair satisfies |w_i^H w_j| <= tol * |w_i| |w_j|.
-// One-sided Jacobi keeps high relative accuracy for small singular values,
-// which the reference's absolute 1e-14 truncation threshold relies on.
+// Method: eigen-decomposition of the reduced density matrix on the side whose
+// factor must be orthonormal.
+//   * 'U' side (the reference's direction 'R': A[site] = U, A[site+1] = S Vh):
+//       rho = theta theta^H (m x m); U_k = top-k eigenvectors of rho;
+//       S Vh = U_k^H theta (one GEMM).
+//   * 'V' side (direction 'L': A[site] = U S, A[site+1] = Vh):
+//       rho = theta^H theta (n x n); V_k = top-k eigenvectors;
+//       U S = theta V_k (one GEMM), Vh = V_k^H.
+// The orthonormal factor comes straight from the eigen-solver (orthonormal to
+// machine precision whatever the spectrum); the other factor is a product,
+// never a division by sigma.  This is the numerical method of the reference's
+// own real SVD: rocsolver_dgesvdj diagonalises A^T A (or A A^T) by Jacobi and
+// takes the singular values from a QR factorisation of A V (rocSOLVER 7.2
+// header, rocsolver-functions.h, GESVDJ).  Here sigma_j is the diagonal of
+// that QR's R, obtained from the Gram matrix of the product factor by a
+// first-order Schur complement (k_qr_sigma), so the reference's absolute
+// cutoff (first sigma < 1e-14, dmrg2_gpu_impl.h:1276 and
+// scalar_traits.h:497-505) sees the same deflated values.
+//
+// Eigen-solver: two-sided block Jacobi.  rho (zero-padded to Np, a multiple
+// of 32) is split into 16-wide blocks; each round pairs all blocks
+// (round-robin) and one workgroup per pair diagonalises its 32x32 Hermitian
+// sub-matrix in LDS by cyclic Jacobi, producing a 32x32 unitary Q_P; a second
+// kernel applies the block-diagonal transform rho <- P^H rho P, V <- V P with
+// MFMA tiles.  Every sweep starts by sorting the diagonal (largest first),
+// which groups the graded DMRG spectrum and cuts the sweep count.  A sweep
+// ends with a device reduction of max |rho_ij| (i != j); the solver stops
+// when it is below tol * |rho|_F.
 #pragma once
 #include "common.hpp"
 
@@ -18,47 +38,52 @@ namespace dmrg {
 
 template <class T>
 struct SvdWork {
-  // Device workspace sized for p, q <= cap.
-  T* W = nullptr;        // cap * capq
-  T* V = nullptr;        // capq * capq
-  T* gram = nullptr;     // partial Gram matrices
-  double* sigma = nullptr;
-  int* perm = nullptr;
-  unsigned long long* off = nullptr;   // max relative off-diagonal of a sweep
+  T* A[2] = {nullptr, nullptr};  // density matrix, ping-pong (capN^2 each)
+  T* V[2] = {nullptr, nullptr};  // accumulated eigenvectors
+  T* Y = nullptr;                // staging for the gathered / product factor
+  T* Q = nullptr;                // per-pair 32x32 rotations of one round
+  int* qflag = nullptr;          // per pair: 1 if its Q is not the identity
+  int* perm = nullptr;           // sort permutation
+  double* sigma = nullptr;       // singular values in output order
+  double* resid = nullptr;       // first-pass QR residuals
+  double* scal = nullptr;        // [0] |rho|_F, [1] discarded weight, [2] trace(rho)
+  unsigned long long* off = nullptr;  // max |rho_ij| (i != j) of a sweep, as bits
   int* newk = nullptr;
-  int cap = 0, capq = 0;
-  size_t gram_elems = 0;
-  void alloc(int cap_rows);
+  int cap = 0;  // largest supported rho dimension (multiple of 32)
+  void alloc(int cap_n);
   void release();
 };
 
-struct SvdResult {
-  int rows_p, cols_q;  // working matrix shape
-  bool transposed;     // working matrix is A^H
-  int sweeps;
-  double off;          // final relative off-diagonal measure
+struct SvdParams {
+  int max_sweeps = 40;  // outer block-Jacobi sweeps
+  double tol = 1e-14;   // stop when max |rho_ij| <= tol * |rho|_F
 };
 
-// Factor A (m x n, column-major, lda) as U S V^H, singular values sorted
-// descending into work.sigma/perm.  Synchronises the stream once per sweep.
-template <class T>
-SvdResult svd_jacobi(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int lda,
-                     int max_sweeps = 40);
+struct SvdResult {
+  int N;             // dimension of the density matrix that was diagonalised
+  int sweeps;        // outer Jacobi sweeps
+  double off;        // final max |rho_ij| / |rho|_F
+  int nk;            // kept rank
+  double discarded;  // sum of the discarded sigma^2 (trace - kept)
+};
 
-// Truncation rank on device: first i < k with sigma_i < max(cutoff, rel*sigma_0),
-// else k, at least 1 (scalar_traits.h:497-505).  Returns it on the host.
+// Truncated split of A (m x n, column-major, lda).
+//   k = min(m, n, kmax); nk = first j < k with sigma_j < max(cutoff,
+//   rel_cutoff * sigma_0), else k; at least 1.
+//   scale_left == false ('U' side): left (m x nk, ldl) = U, right (nk x n,
+//     leading dimension nk) = S Vh.
+//   scale_left == true ('V' side): left = U S, right = Vh.
+//   sigma_out (optional, device): the nk kept singular values.
+// Synchronises the stream once per Jacobi sweep and once for the rank.
 template <class T>
-int svd_rank(hipStream_t st, SvdWork<T>& w, const SvdResult& r, int k, double cutoff,
-             double rel_cutoff);
+SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int lda, int kmax,
+                    double cutoff, double rel_cutoff, bool scale_left, T* left, int ldl, T* right,
+                    double* sigma_out, const SvdParams& prm);
 
-// Write the truncated factors.  left (m x nk, ldl) gets U or U*S; right
-// (nk x n, ldr) gets V^H or S*V^H, depending on scale_left.
+// Eigen-decomposition of a Hermitian matrix H (N x N, ldh): eigenvalues
+// descending into evals (device, N), eigenvectors into evecs (N x N, lde).
 template <class T>
-void svd_write(hipStream_t st, SvdWork<T>& w, const SvdResult& r, int m, int n, int nk,
-               bool scale_left, T* left, int ldl, T* right, int ldr);
-
-// Copy the first `count` sorted singular values to a device buffer.
-template <class T>
-void svd_sigma(hipStream_t st, SvdWork<T>& w, const SvdResult& r, int count, double* out);
+int heig(hipStream_t st, SvdWork<T>& w, const T* H, int N, int ldh, double* evals, T* evecs,
+         int lde, const SvdParams& prm);
 
 }  // namespace dmrg

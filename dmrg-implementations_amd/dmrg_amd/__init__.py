@@ -34,7 +34,7 @@ EXPORTED = [
     "dmrg_run", "dmrg_get_energy", "dmrg_get_sweep_energies", "dmrg_get_bond_dims",
     "dmrg_get_mps", "dmrg_get_env", "dmrg_expectation", "dmrg_get_stats", "dmrg_reset_stats", "dmrg_last_error",
     "dmrg_ctx_create", "dmrg_ctx_destroy", "dmrg_op_heff2", "dmrg_op_heff1", "dmrg_op_env_left",
-    "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
+    "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
     "dmrg_op_gemm", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h",
     "dmrg_dev_synchronize", "dmrg_device_count",
 ]
@@ -50,7 +50,7 @@ class Options(C.Structure):
     _fields_ = [("krylov_max", C.c_int), ("lanczos_tol", C.c_double), ("eig_tol", C.c_double),
                 ("svd_cutoff", C.c_double), ("svd_rel_cutoff", C.c_double),
                 ("precision", C.c_int), ("profile", C.c_int), ("device", C.c_int),
-                ("svd_max_sweeps", C.c_int)]
+                ("svd_max_sweeps", C.c_int), ("svd_tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -104,6 +104,7 @@ def lib():
         "dmrg_op_env_left": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp]),
         "dmrg_op_env_right": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp]),
         "dmrg_op_svd_split": (i, [vp, vp, i, i, i, i, d, d, i, vp, vp, vp, vp, C.POINTER(i)]),
+        "dmrg_op_heig": (i, [vp, vp, i, i, vp, vp, vp, C.POINTER(i)]),
         "dmrg_op_tridiag_lowest": (i, [vp, i, C.POINTER(d), C.POINTER(d), C.POINTER(d),
                                        C.POINTER(d)]),
         "dmrg_op_lanczos2": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp, i, i, C.POINTER(d),
@@ -370,6 +371,17 @@ class Ops:
         lf = left.numpy().ravel(order="F")[: m * nk].reshape(m, nk, order="F")
         rf = right.numpy().ravel(order="F")[: nk * n].reshape(nk, n, order="F")
         return lf, rf, sig.numpy()[:nk], nk
+
+    def heig(self, H):
+        """Hermitian eigen-decomposition of a device matrix H (N x N); returns
+        numpy (evals descending, evecs, sweeps)."""
+        N = H.shape[0]
+        ev = DeviceArray((N,), np.float64)
+        V = DeviceArray((N, N), H.dtype)
+        sw = C.c_int(0)
+        _check(lib().dmrg_op_heig(self._h, None, self._dt(H), N, H.ptr, ev.ptr, V.ptr,
+                                  C.byref(sw)))
+        return ev.numpy(), V.numpy(), sw.value
 
     def gemm(self, opA, opB, A, B, Cm, alpha=1.0, beta=0.0, stream=None):
         """Cm = alpha op(A) op(B) + beta Cm on 2-D device arrays."""

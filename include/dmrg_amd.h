@@ -57,7 +57,8 @@ typedef struct dmrg_options {
   int precision;         /* DMRG_PREC_*                                       */
   int profile;           /* 1: record per-phase HIP-event timings             */
   int device;            /* HIP device ordinal                                */
-  int svd_max_sweeps;    /* Jacobi sweep cap of the SVD                       */
+  int svd_max_sweeps;    /* Jacobi sweep cap of the SVD eigen-solver          */
+  double svd_tol;        /* Jacobi stop: max |rho_ij| <= svd_tol * |rho|_F    */
 } dmrg_options;
 
 typedef struct dmrg_stats {
@@ -174,6 +175,14 @@ int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co
 int dmrg_op_svd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, int k,
                       double cutoff, double rel_cutoff, int scale_left, const void* M,
                       void* left, void* right, double* sigma, int* nk_out);
+/* Hermitian eigen-decomposition H = V diag(evals) V^H of a device matrix
+ * H (N x N, column-major, N <= 2048) by the block-Jacobi solver behind
+ * dmrg_op_svd_split; evals descending (device, N), evecs (device, N x N).
+ * The reference reaches the same solver class through rocsolver_dgesvdj
+ * (scalar_traits.h:166-188), whose V is the Jacobi eigenbasis of A^T A.
+ * *sweeps_out (host, may be NULL) gets the Jacobi sweep count. */
+int dmrg_op_heig(dmrg_ctx_handle c, void* stream, int dtype, int N, const void* H, double* evals,
+                 void* evecs, int* sweeps_out);
 /* Lowest eigenpair of the symmetric tridiagonal (alpha[0..k), beta[0..k-1))
  * on the device -- replaces rocsolver_dsteqr (dmrg2_gpu_impl.h:1110-1125).
  * Host arrays in, host results out. */
