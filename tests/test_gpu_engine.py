@@ -73,3 +73,74 @@ def test_fp32_matvec_mode_tracks_fp64(dmrg_amd):
         assert abs(e - out[prec]) < 1e-9 * L
         eng.close()
     assert abs(out[1] - out[0]) < 1e-9 * L, out
+
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def _fixture(name):
+    import os
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def test_golden_heisenberg_L12_fixture(dmrg_amd):
+    """The reference's benchmark fixture (benchmark_data/regular/heisenberg/
+    L12_D20: quimb MPO + left-canonical initial MPS, chi=20): the engine run
+    from that MPS matches the oracle sweep by sweep and the quimb DMRG2 golden
+    energy, and stays above exact diagonalisation."""
+    z = _fixture("heis_L12_D20.npz")
+    L, chi = 12, int(z["chi"])
+    Ws = [z["W"][i] for i in range(L)]
+    mps = [z["mps_%d" % i] for i in range(L)]
+    bonds = [1] + [A.shape[2] for A in mps]
+    E_o, _, _, sw_o = od.dmrg2(Ws, mps, bonds, chi, 6, tol=0.0, krylov_max=40)
+    eng = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=40)
+    eng.set_mpo(Ws)
+    eng.set_mps(mps)
+    E = eng.run(6)
+    sw = eng.sweep_energies()
+    eng.close()
+    assert np.max(np.abs(np.array(sw) - np.array(sw_o))) < 1e-10 * L
+    assert E >= -5.142090632840533 - 1e-12
+    assert abs(E - float(z["E_dmrg2"])) < 5e-9
+
+
+def test_golden_josephson_L20_fixture(dmrg_amd):
+    """The reference's complex128 Josephson fixture (L=20, n_max=2, E_J=1,
+    E_C=0.5, flux pi/4, chi=50): quimb DMRG2 reached -7.839066448948966 and
+    DMRG1 -7.839066465322219; a two-site run must land in that window."""
+    z = _fixture("josephson_L20_D50_nmax2.npz")
+    L, chi = 20, int(z["chi"])
+    Ws = [z["W"][i] for i in range(L)]
+    eng = dmrg_amd.DMRG(L, 5, chi, 4, dtype="c128", tol=1e-12, krylov_max=40)
+    eng.set_mpo(Ws)
+    eng.initialize_mps_random(seed=20, scale=0.1)
+    E = eng.run(12)
+    e, n2 = eng.expectation()
+    eng.close()
+    assert abs(n2 - 1) < 1e-10 and abs(e - E) < 1e-9
+    assert float(z["E_dmrg1"]) - 1e-8 < E < float(z["E_dmrg2"]) + 5e-9
+
+
+def test_config1_single_site_L12(dmrg_amd):
+    """Single-site, L=12, chi=32, Krylov 20, 10 sweeps from a normalised
+    PCG64(seed 12) random MPS fed to both the engine and the oracle: sweep
+    energies agree to 1e-10 per site; chi=32 < 64 truncates, so the energy
+    stays above exact diagonalisation."""
+    L, d, chi = 12, 2, 32
+    rng = np.random.default_rng(12)
+    bonds = od.bond_dims_init(L, d, chi)
+    mps = [rng.uniform(-1, 1, (bonds[i], d, bonds[i + 1])) for i in range(L)]
+    nrm = np.sqrt(od.norm2(mps))
+    mps[0] = mps[0] / nrm
+    Ws = models.heisenberg_mpo(L)
+    E_o, _, _, sw_o = od.dmrg1(Ws, mps, bonds, chi, 10, tol=0.0, krylov_max=20)
+    eng = dmrg_amd.DMRG(L, d, chi, 5, tol=0.0, variant=1, krylov_max=20)
+    eng.set_mpo(Ws)
+    eng.set_mps(mps)
+    E = eng.run(10)
+    sw = eng.sweep_energies()
+    eng.close()
+    assert np.max(np.abs(np.array(sw) - np.array(sw_o))) < 1e-10 * L
+    E_ed = -5.142090632840533
+    assert E >= E_ed - 1e-12 and E - E_ed < 1e-6

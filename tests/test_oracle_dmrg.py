@@ -1,0 +1,207 @@
+"""CPU: pin the oracle's DMRG primitives and sweeps (no GPU).
+
+Checks follow the reference's own tests where they exist
+(cpu/pdmrg/pdmrg/tests/test_numerics.py) and otherwise what the mathematics
+fixes (dense spectra, exact diagonalisation, variational bounds).
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import dmrg as od
+from oracle import models
+from oracle import pdmrg_serial as ps
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _r(rng, shape, cplx=False):
+    a = rng.standard_normal(shape)
+    return a + 1j * rng.standard_normal(shape) if cplx else a
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_apply_heff2_matches_reference_einsum(cplx):
+    """test_numerics.py:112-126: 'awc,csre,wmsp,mnrq,fne->apqf' in the
+    cpu/pdmrg conventions (L[bra, mpo, ket], W[wl, wr, up, down])."""
+    rng = np.random.default_rng(42)
+    cL, cR, d, D = 3, 4, 2, 5
+    Lp = _r(rng, (cL, D, cL), cplx)        # pdmrg L[a(bra), w, c(ket)]
+    Rp = _r(rng, (cR, D, cR), cplx)
+    WLp = _r(rng, (D, D, d, d))            # pdmrg W[w, m, s, p]
+    WRp = _r(rng, (D, D, d, d))
+    thp = _r(rng, (cL, d, d, cR), cplx)    # theta[c, s, r, e]
+    ref = np.einsum("awc,csre,wmsp,mnrq,fne->apqf", Lp, thp, WLp, WRp, Rp, optimize=True)
+    # engine conventions: env (ket, mpo, bra) = pdmrg transposed; W[w, s(ket), sp(bra), wp]
+    Lg, Rg = Lp.transpose(2, 1, 0), Rp.transpose(2, 1, 0)
+    W1, W2 = WLp.transpose(0, 2, 3, 1), WRp.transpose(0, 2, 3, 1)
+    got = od.apply_heff2(Lg, Rg, W1, W2, thp)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    # the restated pdmrg routine agrees with its own test
+    got2 = ps.apply_heff(Lp, Rp, WLp, WRp, thp)
+    assert np.abs(got2 - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_update_envs_match_reference_einsums():
+    """test_numerics.py:162-192."""
+    rng = np.random.default_rng(42)
+    chi, chi2, D, d = 3, 4, 5, 2
+    Lp = _r(rng, (chi, D, chi))
+    A = _r(rng, (chi, d, chi2))
+    Wp = _r(rng, (D, D, d, d))
+    refL = np.einsum("awc,atp,wmst,csq->pmq", Lp, A.conj(), Wp, A, optimize=True)
+    W = Wp.transpose(0, 2, 3, 1)
+    gotL = od.update_left_env(Lp.transpose(2, 1, 0), A, W)          # (ket, mpo, bra)
+    assert np.abs(gotL.transpose(2, 1, 0) - refL).max() < 1e-12
+    assert np.abs(ps.update_left_env(Lp, A, Wp) - refL).max() < 1e-12
+    Rp = _r(rng, (chi, D, chi))
+    B = _r(rng, (chi2, d, chi))
+    refR = np.einsum("awc,pta,mwst,qsc->pmq", Rp, B.conj(), Wp, B, optimize=True)
+    gotR = od.update_right_env(Rp.transpose(2, 1, 0), B, W)
+    assert np.abs(gotR.transpose(2, 1, 0) - refR).max() < 1e-12
+    assert np.abs(ps.update_right_env(Rp, B, Wp) - refR).max() < 1e-12
+
+
+def test_heff2_spectrum_equals_dense_4site():
+    """test_numerics.py:73-110: with identity-like right sites, H_eff on
+    sites (0,1) of a 4-site chain has the spectrum of the dense H."""
+    L, d = 4, 2
+    Ws = models.heisenberg_mpo(L)
+    A2 = np.zeros((4, 2, 2))
+    for s in range(2):
+        for b in range(2):
+            A2[2 * s + b, s, b] = 1.0
+    A3 = np.zeros((2, 2, 1))
+    for s in range(2):
+        A3[s, s, 0] = 1.0
+    D = Ws[0].shape[0]
+    R = od.right_boundary(D, float)
+    R = od.update_right_env(R, A3, Ws[3])
+    R = od.update_right_env(R, A2, Ws[2])
+    Lb = od.left_boundary(D, float)
+    n = 1 * d * d * 4
+    H = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        H[:, j] = od.apply_heff2(Lb, R, Ws[0], Ws[1], e.reshape(1, d, d, 4, order="F")).ravel(order="F")
+    assert np.abs(H - H.T).max() < 1e-14
+    ev = np.sort(np.linalg.eigvalsh(H))
+    ref = np.sort(np.linalg.eigvalsh(models.dense_heisenberg(4).toarray()))
+    assert np.abs(ev - ref).max() < 1e-12
+
+
+def test_product_state_energy():
+    """test_numerics.py:212-227: all-up product state has E = (L-1)/4."""
+    L = 8
+    mps, _ = od.product_mps(L, 2)
+    assert abs(od.expectation(mps, models.heisenberg_mpo(L)) - (L - 1) / 4) < 1e-14
+
+
+def test_expectation_equals_dense():
+    L = 6
+    mps, _ = od.random_mps(L, 2, 8, seed=9)
+    psi = mps[0]
+    for A in mps[1:]:
+        psi = np.tensordot(psi, A, axes=([psi.ndim - 1], [0]))
+    psi = psi.reshape(-1, order="F")
+    H = models.dense_heisenberg(L).toarray()
+    # Fortran flattening puts site 0 fastest; the Kronecker builder puts site 0
+    # slowest, so reverse the site order of the dense vector.
+    psi = psi.reshape([2] * L, order="F").transpose(list(range(L - 1, -1, -1))).reshape(-1, order="F")
+    E_dense = psi @ H @ psi / (psi @ psi)
+    E = od.expectation(mps, models.heisenberg_mpo(L)) / od.norm2(mps)
+    assert abs(E - E_dense) < 1e-12
+
+
+def test_lanczos_lowest_eigenvalue_and_residual():
+    rng = np.random.default_rng(0)
+    n = 120
+    X = rng.standard_normal((n, n))
+    H = (X + X.T) / 2
+    v0 = rng.standard_normal((n,))
+    stats = {}
+    E, v = od.lanczos(lambda x: H @ x, v0, 0, krylov_max=n, stats=stats)
+    ev = np.linalg.eigvalsh(H)
+    assert abs(E - ev[0]) < 1e-10
+    assert np.linalg.norm(H @ v - E * v) < 1e-6
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-12
+
+
+def test_lanczos_invariant_subspace_breakdown():
+    # start vector inside a 2-dimensional invariant subspace: exact after 2 steps
+    H = np.diag([1.0, 2.0, 3.0, 4.0, 5.0])
+    v0 = np.array([1.0, 1.0, 0, 0, 0])
+    E, v = od.lanczos(lambda x: H @ x, v0, 0, krylov_max=5, stats={})
+    assert abs(E - 1.0) < 1e-14
+
+
+def test_svd_split2_truncation_rule():
+    """accurate_svd.py:82-110 keeps min(k, max_bond); the GPU engine also
+    drops sigma < 1e-14 (dmrg2_gpu_impl.h:1276)."""
+    rng = np.random.default_rng(1)
+    cL, d, cR = 6, 2, 6
+    U = np.linalg.qr(rng.standard_normal((cL * d, cL * d)))[0]
+    V = np.linalg.qr(rng.standard_normal((d * cR, d * cR)))[0]
+    s = np.array([1, 0.5, 0.1, 1e-3, 1e-8, 1e-13, 5e-15, 1e-16, 0, 0, 0, 0])
+    M = (U * s) @ V.T
+    th = M.reshape(cL, d, d, cR, order="F")
+    Al, Ar, S, disc = od.svd_split2(th, 100, "R")
+    assert Al.shape[2] == 6            # first sigma below 1e-14 is index 6
+    Al, Ar, S, disc = od.svd_split2(th, 3, "L")
+    assert Al.shape[2] == 3
+    assert abs(disc - np.sum(s[3:] ** 2)) < 1e-15
+    rec = np.tensordot(Al, Ar, axes=([2], [0])).reshape(cL * d, d * cR, order="F")
+    assert np.linalg.norm(rec - (U[:, :3] * s[:3]) @ V[:, :3].T) < 1e-12
+
+
+@pytest.mark.parametrize("L,E", [(4, -1.616025403784), (8, -3.374932598688)])
+def test_dmrg2_known_answers(L, E):
+    mps, b = od.neel_mps(L, 2, 32)
+    Ef, _, _, _ = od.dmrg2(models.heisenberg_mpo(L), mps, b, 32, 20, krylov_max=20)
+    assert abs(Ef - E) < 1e-10
+
+
+def test_dmrg1_known_answer_random_start():
+    L = 8
+    mps, b = od.random_mps(L, 2, 32, seed=4, scale=0.1)
+    Ef, _, _, _ = od.dmrg1(models.heisenberg_mpo(L), mps, b, 32, 30, krylov_max=20)
+    assert abs(Ef - (-3.374932598688)) < 1e-10
+
+
+def test_dmrg2_josephson_known_answer():
+    L = 6
+    Ws = models.josephson_mpo(L, n_max=1, E_J=1.0, E_C=0.5, phi_ext=np.pi / 4)
+    mps, b = od.random_mps(L, 3, 64, seed=6, complex_=True, scale=0.1)
+    Ef, _, _, _ = od.dmrg2(Ws, mps, b, 64, 20, krylov_max=40)
+    assert abs(Ef - (-1.748843818181493)) < 1e-10
+
+
+def test_dmrg2_golden_L12_from_reference_fixture():
+    """benchmark_data/regular/heisenberg/L12_D20: the reference's MPO and
+    initial MPS, chi=20; quimb DMRG2 golden energy -5.142090628178122 and the
+    exact ground energy -5.142090632840533 (chi=20 truncates)."""
+    z = np.load(os.path.join(GOLDEN, "heis_L12_D20.npz"), allow_pickle=False)
+    L = 12
+    Ws = [z["W"][i] for i in range(L)]
+    mps = [z["mps_%d" % i] for i in range(L)]
+    bonds = [1] + [A.shape[2] for A in mps]
+    rec = {}
+    Ef, _, _, sweeps = od.dmrg2(Ws, mps, bonds, int(z["chi"]), 20, tol=1e-12, krylov_max=40,
+                                record=rec)
+    E_ed = -5.142090632840533
+    assert Ef >= E_ed - 1e-12                       # variational
+    assert abs(Ef - float(z["E_dmrg2"])) < 5e-9     # same truncation-limited optimum
+
+
+@pytest.mark.parametrize("L", [10])
+def test_pdmrg_serial_matches_oracle(L):
+    """The restated cpu/pdmrg serial two-site sweep (the timed CPU baseline)
+    reaches the oracle's / ED energy."""
+    Ws = models.heisenberg_mpo(L)
+    mps, _ = od.random_mps(L, 2, 32, seed=L, scale=0.1)
+    E, _ = ps.serial_dmrg([np.array(A) for A in mps], [ps.to_pdmrg_mpo(W) for W in Ws], 32, 6)
+    E_ed = models.ground_energy(models.dense_heisenberg(L))
+    assert abs(E - E_ed) < 1e-9
