@@ -121,6 +121,9 @@ def main():
     ap.add_argument("--cpu-bonds", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="before the runs, copy MIB MiB device-to-device 3 times (the known-size "
+                         "copy kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; the DMRG sweep of a single chain does not "
@@ -128,6 +131,15 @@ def main():
         sys.exit(2)
     import dmrg_amd
     wl = WORKLOADS[a.workload]
+    if a.calib_copy > 0:
+        import numpy as np
+        n = a.calib_copy * (1 << 20) // 8
+        src, dst = dmrg_amd.DeviceArray((n,), np.float64), dmrg_amd.DeviceArray((n,), np.float64)
+        for _ in range(3):
+            dmrg_amd.copy_device(dst, src)
+        dmrg_amd.synchronize()
+        src.free()
+        dst.free()
     modes = a.modes.split(",")
     res, engines = {}, {}
     for m in modes:

@@ -1,8 +1,13 @@
 // extern "C" boundary (include/dmrg_amd.h).  No exception crosses it: every
 // entry point returns a status code and records the message per thread.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/dmrg_amd.h"
@@ -100,6 +105,60 @@ int dmrg_create(int L, int d, int chi_max, int D, double tol, int dtype, int var
 }
 
 void dmrg_destroy(dmrg_handle h) { delete h; }
+
+int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* energies,
+                   double* seconds) {
+  if (!hs || !energies || n < 0) return fail(DMRG_ERR_BAD_ARG, "dmrg_run_batch: bad argument");
+  for (int i = 0; i < n; ++i)
+    if (!hs[i]) return fail(DMRG_ERR_BAD_ARG, "dmrg_run_batch: NULL engine");
+  return guard([&] {
+    // longest first: estimated cost L * chi^3 * d^3 per sweep
+    std::vector<int> order(n);
+    std::vector<double> cost(n);
+    for (int i = 0; i < n; ++i) {
+      order[i] = i;
+      hs[i]->with([&](auto& e) {
+        cost[i] = (double)e.L * e.chi * (double)e.chi * e.chi * e.d * e.d * e.d;
+      });
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::atomic<int> next{0};
+    std::atomic<bool> stop{false};
+    std::mutex mu;
+    int err_code = 0;
+    std::string err_msg;
+    auto worker = [&]() {
+      for (;;) {
+        if (stop.load()) return;
+        const int j = next.fetch_add(1);
+        if (j >= n) return;
+        const int i = order[j];
+        try {
+          const auto t0 = std::chrono::steady_clock::now();
+          double E = 0.0;
+          hs[i]->with([&](auto& e) { E = e.run(n_sweeps, true); });
+          energies[i] = E;
+          if (seconds)
+            seconds[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        } catch (const Error& e) {
+          std::lock_guard<std::mutex> lk(mu);
+          if (!err_code) { err_code = e.code; err_msg = "chain " + std::to_string(i) + ": " + e.what(); }
+          stop = true;
+        } catch (const std::exception& e) {
+          std::lock_guard<std::mutex> lk(mu);
+          if (!err_code) { err_code = ST_HIP; err_msg = "chain " + std::to_string(i) + ": " + e.what(); }
+          stop = true;
+        }
+      }
+    };
+    const int nt = std::max(1, std::min(threads, n));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    if (err_code) throw Error(err_code, err_msg);
+  });
+}
 
 int dmrg_set_mpo(dmrg_handle h, const void* const* W) {
   if (!h || !W) return fail(DMRG_ERR_BAD_ARG, "dmrg_set_mpo: NULL argument");
@@ -516,6 +575,9 @@ int dmrg_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 }
 int dmrg_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return guard([&] { DMRG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int dmrg_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  return guard([&] { DMRG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
 }
 int dmrg_dev_synchronize(void) {
   return guard([&] { DMRG_HIP(hipDeviceSynchronize()); });

@@ -552,6 +552,7 @@ template <class T>
 double Engine<T>::run(int n_sweeps, bool quiet) {
   if (!mpo_set) throw Error(ST_STATE, "run: set_mpo has not been called");
   DMRG_REQUIRE(n_sweeps >= 0, "run: negative sweep count");
+  DMRG_HIP(hipSetDevice(opt.device));  // the calling thread may not be the creating one
   double te = now_s();
   build_envs();
   double t_start = now_s();

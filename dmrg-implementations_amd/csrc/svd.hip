@@ -150,66 +150,74 @@ __global__ void k_permute(const T* __restrict__ A, const T* __restrict__ V, T* _
 // c = 1/sqrt(1+t^2), s = t c, J = [[c, s e^{i phi}], [-s e^{-i phi}, c]];
 // J^H [[a_aa, beta], [conj(beta), a_bb]] J = diag(a_aa - t|beta|, a_bb + t|beta|)
 // (the symmetric Schur decomposition with a phase, Golub & Van Loan 8.4).
+//
+// One wave per sub-problem: the 64 lanes synchronise with wave barriers only
+// (LDS operations of one wave complete in program order), so an inner round
+// costs no workgroup barrier.  The 2x2-block and Q updates are branch-free:
+// a pair that is not rotated carries (c, s) = (1, 0), an exact identity.
+constexpr int NTA = 64;
+
 template <class T>
-__global__ __launch_bounds__(NT) void k_pair_eig(T* __restrict__ A, int ld, int nb, int round,
-                                                 T* __restrict__ Qout, int* __restrict__ qflag,
-                                                 const double* __restrict__ scal, double rot_tol,
-                                                 int inner_max) {
+__global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int nb, int round,
+                                                  T* __restrict__ Qout, int* __restrict__ qflag,
+                                                  const double* __restrict__ scal, double rot_tol,
+                                                  int inner_max) {
   __shared__ T As[JP * LDP];  // As[c * LDP + r] = sub(r, c)
   __shared__ T Qs[JP * LDP];
   __shared__ double rc[JP / 2];
   __shared__ T rs[JP / 2];
-  __shared__ int rf[JP / 2];
-  __shared__ int s_rot, s_any;
+  __shared__ int s_rot;
   const int k = blockIdx.x, t = threadIdx.x;
   int I, J;
   rr_pair(nb, round, k, I, J);
   const double thr = rot_tol * scal[0];
   const double thr2 = thr * thr;
   int big = 0;
-  for (int e = t; e < JP * JP; e += NT) {
+  for (int e = t; e < JP * JP; e += NTA) {
     const int r = e % JP, c = e / JP;
     const T v = A[(size_t)gidx(I, J, r) + (size_t)gidx(I, J, c) * ld];
     As[c * LDP + r] = v;
     Qs[c * LDP + r] = (r == c) ? T(1) : T(0);
-    if (r != c && absv(v) > thr) big = 1;
+    if (r != c && abs2(v) > thr2) big = 1;
   }
-  if (t == 0) { s_rot = 0; s_any = 0; }
-  if (!__syncthreads_or(big)) {
+  if (__ballot(big) == 0) {
     if (t == 0) qflag[k] = 0;
     return;
   }
+  if (t == 0) s_rot = 0;
+  __syncwarp();
   // Circle-schedule indices, advanced incrementally each round (index 0 is
   // fixed; the others step 1 -> 2 -> ... -> JP-1 -> 1), starting from
   // position == index.  After JP-1 rounds they are back at the start.
   auto adv = [](int i) { return i == 0 ? 0 : (i == JP - 1 ? 1 : i + 1); };
-  // HP = JP/2 pairs per round.  Thread t updates the 2x2 blocks of row pair
-  // k1 = t / TPR and column pairs k2 = t % TPR + TPR*q (TPR = NT/HP threads
+  // HP = JP/2 pairs per round.  Lane t updates the 2x2 blocks of row pair
+  // k1 = t / TPR and column pairs k2 = t % TPR + TPR*q (TPR = NTA/HP lanes
   // per row pair), and the Q entries of row t % JP in pairs
-  // kk = t / JP + (NT/JP)*q.
+  // kk = t / JP + (NTA/JP)*q.
   constexpr int HP = JP / 2;
-  constexpr int TPR = NT / HP;
-  constexpr int NBQ = HP / TPR, NQQ = JP * HP / NT;
-  static_assert(NT % HP == 0 && HP % TPR == 0 && NT % JP == 0 && (JP * HP) % NT == 0, "mapping");
+  constexpr int TPR = NTA / HP;
+  constexpr int NBQ = HP / TPR, NQQ = JP * HP / NTA;
+  static_assert(NTA % HP == 0 && HP % TPR == 0 && NTA % JP == 0 && (JP * HP) % NTA == 0,
+                "mapping");
   const int k1 = t / TPR;
   int a1 = k1, b1 = JP - 1 - k1;
   int a2[NBQ], b2[NBQ], aq[NQQ], bq[NQQ];
 #pragma unroll
   for (int q = 0; q < NBQ; ++q) { a2[q] = t % TPR + TPR * q; b2[q] = JP - 1 - a2[q]; }
 #pragma unroll
-  for (int q = 0; q < NQQ; ++q) { aq[q] = t / JP + (NT / JP) * q; bq[q] = JP - 1 - aq[q]; }
+  for (int q = 0; q < NQQ; ++q) { aq[q] = t / JP + (NTA / JP) * q; bq[q] = JP - 1 - aq[q]; }
   int ap = t % HP, bp = JP - 1 - (t % HP);
   const int row = t % JP;
+  int any = 0;
   for (int sw = 0; sw < inner_max; ++sw) {
     for (int rd = 0; rd < JP - 1; ++rd) {
-      if (t < JP / 2) {
+      if (t < HP) {
         const int a = ap, b = bp;
         const double al = real(As[a * LDP + a]), ga = real(As[b * LDP + b]);
         const T be = As[b * LDP + a];
         const double mag2 = abs2(be);
         double c = 1.0;
         T sp = T(0);
-        int f = 0;
         if (mag2 > thr2) {
           // t = sign(tau)/(|tau| + sqrt(1+tau^2)) with tau = dl/(2|beta|), written
           // as t = 2|beta| sgn w, w = 1/(|dl| + sqrt(dl^2 + 4|beta|^2)), so one
@@ -224,22 +232,19 @@ __global__ __launch_bounds__(NT) void k_pair_eig(T* __restrict__ A, int ld, int 
           As[b * LDP + b] = T(ga + tb);
           As[b * LDP + a] = T(0);
           As[a * LDP + b] = T(0);
-          f = 1;
           s_rot = 1;
         }
         rc[t] = c;
         rs[t] = sp;
-        rf[t] = f;
       }
-      __syncthreads();
+      __syncwarp();
       // Off-diagonal 2x2 blocks: X <- J1^H X J2.
-      const int f1 = rf[k1];
       const double c1 = rc[k1];
       const T s1 = rs[k1];
 #pragma unroll
       for (int q = 0; q < NBQ; ++q) {
         const int k2 = t % TPR + TPR * q;
-        if (k1 == k2 || !(f1 | rf[k2])) continue;
+        if (k1 == k2) continue;
         const T x00 = As[a2[q] * LDP + a1], x01 = As[b2[q] * LDP + a1];
         const T x10 = As[a2[q] * LDP + b1], x11 = As[b2[q] * LDP + b1];
         const double c2 = rc[k2];
@@ -254,8 +259,7 @@ __global__ __launch_bounds__(NT) void k_pair_eig(T* __restrict__ A, int ld, int 
       // Q <- Q J.
 #pragma unroll
       for (int q = 0; q < NQQ; ++q) {
-        const int kk = t / JP + (NT / JP) * q;
-        if (!rf[kk]) continue;
+        const int kk = t / JP + (NTA / JP) * q;
         const T qa = Qs[aq[q] * LDP + row], qb = Qs[bq[q] * LDP + row];
         const double c = rc[kk];
         const T s = rs[kk];
@@ -270,18 +274,18 @@ __global__ __launch_bounds__(NT) void k_pair_eig(T* __restrict__ A, int ld, int 
       for (int q = 0; q < NQQ; ++q) { aq[q] = adv(aq[q]); bq[q] = adv(bq[q]); }
       ap = adv(ap);
       bp = adv(bp);
-      __syncthreads();
+      __syncwarp();
     }
     const int rot = s_rot;
-    __syncthreads();
-    if (t == 0) { s_any |= rot; s_rot = 0; }
-    __syncthreads();
+    __syncwarp();
+    if (t == 0) s_rot = 0;
+    __syncwarp();
+    any |= rot;
     if (!rot) break;
   }
-  const int any = s_any;
   if (t == 0) qflag[k] = any;
   if (!any) return;
-  for (int e = t; e < JP * JP; e += NT) {
+  for (int e = t; e < JP * JP; e += NTA) {
     const int r = e % JP, c = e / JP;
     A[(size_t)gidx(I, J, r) + (size_t)gidx(I, J, c) * ld] = As[c * LDP + r];
     Qout[(size_t)k * JP * JP + e] = Qs[c * LDP + r];
@@ -584,7 +588,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
                        w.A[1 - cur], w.V[1 - cur], w.perm, Np);
     cur = 1 - cur;
     for (int r = 0; r < nb - 1; ++r) {
-      hipLaunchKernelGGL(k_pair_eig<T>, dim3(np), dim3(NT), 0, st, w.A[cur], Np, nb, r, w.Q,
+      hipLaunchKernelGGL(k_pair_eig<T>, dim3(np), dim3(NTA), 0, st, w.A[cur], Np, nb, r, w.Q,
                          w.qflag, (const double*)w.scal, rot_tol, inner_max);
       hipLaunchKernelGGL(k_pair_apply<T>, dim3(noff + nvt), dim3(NT), 0, st, w.A[cur], w.V[cur], Np,
                          nb, r, (const T*)w.Q, (const int*)w.qflag);

@@ -31,11 +31,11 @@ PREC_FP64, PREC_FP32 = 0, 1
 EXPORTED = [
     "dmrg_default_options", "dmrg_create", "dmrg_destroy", "dmrg_set_mpo",
     "dmrg_init_mps_random", "dmrg_init_mps_product", "dmrg_init_mps_neel", "dmrg_set_mps",
-    "dmrg_run", "dmrg_get_energy", "dmrg_get_sweep_energies", "dmrg_get_bond_dims",
+    "dmrg_run", "dmrg_run_batch", "dmrg_get_energy", "dmrg_get_sweep_energies", "dmrg_get_bond_dims",
     "dmrg_get_mps", "dmrg_get_env", "dmrg_expectation", "dmrg_get_stats", "dmrg_reset_stats", "dmrg_last_error",
     "dmrg_ctx_create", "dmrg_ctx_destroy", "dmrg_op_heff2", "dmrg_op_heff1", "dmrg_op_env_left",
     "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
-    "dmrg_op_gemm", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h",
+    "dmrg_op_gemm", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h", "dmrg_memcpy_d2d",
     "dmrg_dev_synchronize", "dmrg_device_count",
 ]
 
@@ -88,6 +88,7 @@ def lib():
         "dmrg_init_mps_neel": (i, [vp]),
         "dmrg_set_mps": (i, [vp, i, vp, i, i]),
         "dmrg_run": (i, [vp, i, i, C.POINTER(d)]),
+        "dmrg_run_batch": (i, [vp, i, i, i, vp, vp]),
         "dmrg_get_energy": (i, [vp, C.POINTER(d)]),
         "dmrg_get_sweep_energies": (i, [vp, C.POINTER(d), i, C.POINTER(i)]),
         "dmrg_get_bond_dims": (i, [vp, C.POINTER(i)]),
@@ -114,6 +115,7 @@ def lib():
         "dmrg_dev_free": (i, [vp]),
         "dmrg_memcpy_h2d": (i, [vp, vp, C.c_size_t]),
         "dmrg_memcpy_d2h": (i, [vp, vp, C.c_size_t]),
+        "dmrg_memcpy_d2d": (i, [vp, vp, C.c_size_t]),
         "dmrg_dev_synchronize": (i, []),
         "dmrg_device_count": (i, [C.POINTER(i)]),
     }
@@ -415,6 +417,25 @@ class Ops:
                                       W1.ptr, W2.ptr, theta.ptr, krylov_max, site, C.byref(e),
                                       C.byref(nit)))
         return e.value, nit.value
+
+
+def run_batch(engines, n_sweeps, threads=8):
+    """Run independent DMRG engines concurrently (dmrg_run_batch); returns
+    (energies, seconds) as numpy arrays in the order of `engines`."""
+    import numpy as _np
+    n = len(engines)
+    hs = (C.c_void_p * max(n, 1))(*[e._h.value for e in engines])
+    E = _np.zeros(max(n, 1))
+    T = _np.zeros(max(n, 1))
+    _check(lib().dmrg_run_batch(hs, n, n_sweeps, threads, E.ctypes.data, T.ctypes.data))
+    return E[:n], T[:n]
+
+
+def copy_device(dst, src):
+    """Device-to-device copy of src into dst (same byte size)."""
+    if dst.nbytes != src.nbytes:
+        raise ValueError("copy_device: size mismatch")
+    _check(lib().dmrg_memcpy_d2d(dst.ptr, src.ptr, src.nbytes))
 
 
 def synchronize():

@@ -119,6 +119,17 @@ int dmrg_set_mps(dmrg_handle h, int site, const void* A, int cL, int cR);
  * %.3f s", "  env_build_sec: %.3f  timer_scope: sweep_only"). */
 int dmrg_run(dmrg_handle h, int n_sweeps, int quiet, double* energy);
 
+/* Batched solve of independent chains (no reference counterpart; the
+ * reference runs chains one binary invocation at a time).  hs[0..n) are
+ * engines created and set up as usual (any mix of L, chi, d, D, variant and
+ * precision); each runs dmrg_run(n_sweeps, quiet) on its own streams, and
+ * `threads` host threads feed them to the GPU concurrently, longest chain
+ * first, so the latency-bound phases of different chains overlap.
+ * energies[i] (and seconds[i], may be NULL) get chain i's result.  The first
+ * error stops the remaining chains and is returned. */
+int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* energies,
+                   double* seconds);
+
 /* Replaces get_energy() (dmrg2_gpu.h:40). */
 int dmrg_get_energy(dmrg_handle h, double* energy);
 /* Energies at the end of each completed sweep (returned count in *n). */
@@ -207,6 +218,9 @@ int dmrg_dev_malloc(void** p, size_t bytes);
 int dmrg_dev_free(void* p);
 int dmrg_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int dmrg_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy (a copy kernel of known size; the traffic
+ * calibration of the profiling runs uses it). */
+int dmrg_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int dmrg_dev_synchronize(void);
 int dmrg_device_count(int* n);
 
