@@ -53,6 +53,7 @@ Options to_opts(const dmrg_options* o) {
     r.device = o->device;
     r.svd_max_sweeps = o->svd_max_sweeps;
     r.svd_tol = o->svd_tol;
+    r.svd_warm = o->svd_warm;
   }
   return r;
 }
@@ -85,6 +86,7 @@ void dmrg_default_options(dmrg_options* o) {
   o->device = d.device;
   o->svd_max_sweeps = d.svd_max_sweeps;
   o->svd_tol = d.svd_tol;
+  o->svd_warm = d.svd_warm;
 }
 
 const char* dmrg_last_error(void) { return g_err.c_str(); }
@@ -258,6 +260,7 @@ int dmrg_get_stats(dmrg_handle h, dmrg_stats* st) {
       st->max_discarded = s.max_discarded;
       st->env_build_s = s.env_build_s;
       st->sweeps_s_total = s.sweeps_s_total;
+      st->svd_warm_starts = s.svd_warm_starts;
     });
   });
 }

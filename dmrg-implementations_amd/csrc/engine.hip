@@ -81,6 +81,16 @@ double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int s
 // ---------------------------------------------------------------------------
 // Construction (dmrg2_gpu_impl.h:51-227)
 // ---------------------------------------------------------------------------
+namespace {
+void dmalloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(ST_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+}
+}  // namespace
+
 template <class T>
 Engine<T>::Engine(int L_, int d_, int chi_, int D_, double tol_, int variant_, const Options& o)
     : L(L_), d(d_), chi(chi_), D(D_), tol(tol_), variant(variant_), opt(o) {
@@ -123,13 +133,6 @@ void Engine<T>::alloc_all() {
   const int dv = (variant == 2) ? d * d : d;
   nmax = (size_t)chi * dv * chi;
   kmax = std::min<size_t>((size_t)opt.krylov_max, nmax);
-  auto dmalloc = [](void** p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw Error(ST_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
-    }
-  };
   mps.assign(L, nullptr);
   for (int i = 0; i < L; ++i) dmalloc((void**)&mps[i], site_sz * sizeof(T));
   dmalloc((void**)&mps_spare, site_sz * sizeof(T));
@@ -175,6 +178,7 @@ void Engine<T>::alloc_all() {
   const size_t nb = (size_t)lz_blocks(nmax);
   lz.alloc(std::max(kmax, 1), (size_t)(kmax + 1) * 2 * nb * sizeof(double), sizeof(T));
   svdw.alloc(chi * d);
+  if (variant == 2 && opt.svd_warm) warm_alloc();
   DMRG_HIP(hipHostMalloc((void**)&h_status, 4 * sizeof(int), hipHostMallocDefault));
   DMRG_HIP(hipHostMalloc((void**)&h_scal, 8 * sizeof(double), hipHostMallocDefault));
   runner.lz = &lz;
@@ -208,6 +212,7 @@ void Engine<T>::free_all() {
   sc_f.X = sc_f.Y = nullptr; sc_main.X = sc_main.Y = sc_env.X = sc_env.Y = nullptr;
   lz.release();
   svdw.release();
+  warm_free();
   for (auto e : runner.ev) (void)hipEventDestroy(e);
   runner.ev.clear();
   if (h_status) (void)hipHostFree(h_status);
@@ -400,14 +405,128 @@ void Engine<T>::split_two(int site, char dir) {
   const int m = cL * d, n = d * cR;
   const int full_k = std::min(m, n), k = std::min(full_k, chi);
   double t0 = now_s();
+  const int wi = dir == 'R' ? 0 : 1;
+  const T* warm = warm_on ? warm_basis(site, dir) : nullptr;
+  T* keep = warm_on ? wX[wi][site] : nullptr;
   SvdResult r = svd_split<T>(s_main, svdw, theta, m, n, m, k, opt.svd_cutoff, opt.svd_rel_cutoff,
-                             dir == 'L', mps[site], m, mps[site + 1], nullptr, svd_params());
+                             dir == 'L', mps[site], m, mps[site + 1], nullptr, svd_params(), warm,
+                             keep);
+  if (warm_on) {
+    wXN[wi][site] = dir == 'R' ? m : n;
+    wXB[wi][site] = dir == 'R' ? cL : cR;
+  }
   bond[site + 1] = r.nk;
   DMRG_HIP(hipEventRecord(ev_canon, s_main));
+  if (warm_on) warm_after_split(site, dir);
   stats.svd_ms += 1e3 * (now_s() - t0);
   stats.svd_calls += 1;
   stats.svd_sweeps += r.sweeps;
+  stats.svd_warm_starts += warm ? 1 : 0;
   stats.max_discarded = std::max(stats.max_discarded, r.discarded);
+}
+
+// ---------------------------------------------------------------------------
+// Warm start of the truncation SVD (see warm.hpp)
+// ---------------------------------------------------------------------------
+template <class T>
+void Engine<T>::warm_alloc() {
+  const size_t capN = (size_t)chi * d;
+  const size_t asz = (size_t)chi * d * chi;
+  for (int w = 0; w < 2; ++w) {
+    wX[w].assign(L - 1, nullptr);
+    wXN[w].assign(L - 1, 0);
+    wXB[w].assign(L - 1, 0);
+    for (int i = 0; i < L - 1; ++i) dmalloc((void**)&wX[w][i], capN * capN * sizeof(T));
+    wA[w].assign(L, nullptr);
+    wAL[w].assign(L, 0);
+    wAR[w].assign(L, 0);
+    for (int i = 0; i < L; ++i) dmalloc((void**)&wA[w][i], asz * sizeof(T));
+  }
+  const size_t tsz = (size_t)chi * chi;
+  for (T** p : {&wT[0], &wT[1], &wZ0, &wZ1, &wP, &wTmp}) dmalloc((void**)p, tsz * sizeof(T));
+  dmalloc((void**)&wXw, capN * capN * sizeof(T));
+  dmalloc((void**)&wDev, sizeof(double));
+  warm_on = true;
+}
+
+template <class T>
+void Engine<T>::warm_free() {
+  auto f = [](void* p) { if (p) (void)hipFree(p); };
+  for (int w = 0; w < 2; ++w) {
+    for (auto p : wX[w]) f(p);
+    for (auto p : wA[w]) f(p);
+    wX[w].clear();
+    wA[w].clear();
+  }
+  for (T** p : {&wT[0], &wT[1], &wZ0, &wZ1, &wP, &wTmp, &wXw}) {
+    f(*p);
+    *p = nullptr;
+  }
+  f(wDev);
+  wDev = nullptr;
+  warm_on = false;
+}
+
+// Start of a half-sweep: the empty block overlaps itself, T = [1].
+template <class T>
+void Engine<T>::warm_begin() {
+  if (!warm_on) return;
+  const T one = T(1);
+  DMRG_HIP(hipMemcpyAsync(wT[wCur], &one, sizeof(T), hipMemcpyHostToDevice, s_main));
+  DMRG_HIP(hipStreamSynchronize(s_main));
+  wTr = wTc = 1;
+}
+
+template <class T>
+const T* Engine<T>::warm_basis(int site, char dir) {
+  const int wi = dir == 'R' ? 0 : 1;
+  const int cb = dir == 'R' ? bond[site] : bond[site + 2];
+  const int N = cb * d;
+  if (wXN[wi][site] != N || wXB[wi][site] != cb || wTr != cb || wTc != cb) return nullptr;
+  T* Z = nullptr;
+  if (!polar_unitary<T>(s_main, wT[wCur], cb, wZ0, wZ1, wP, wDev, 0.25, &Z)) return nullptr;
+  if (dir == 'R') warm_basis_left<T>(s_main, Z, cb, d, wX[wi][site], wXw);
+  else warm_basis_right<T>(s_main, Z, cb, d, wX[wi][site], wXw);
+  return wXw;
+}
+
+// After a split: extend the block overlap by the new canonical tensor and
+// keep that tensor for the next sweep.
+template <class T>
+void Engine<T>::warm_after_split(int site, char dir) {
+  if (dir == 'R') {
+    const int cL = bond[site], nk = bond[site + 1];
+    const T* Anew = mps[site];
+    if (wTr > 0 && wAL[0][site] == wTr && wTc == cL) {
+      overlap_left_step<T>(s_main, wT[wCur], wTr, wTc, wA[0][site], wAR[0][site], Anew, nk, d,
+                           wT[1 - wCur], wTmp);
+      wCur = 1 - wCur;
+      wTr = wAR[0][site];
+      wTc = nk;
+    } else {
+      wTr = wTc = 0;
+    }
+    DMRG_HIP(hipMemcpyAsync(wA[0][site], Anew, (size_t)cL * d * nk * sizeof(T),
+                            hipMemcpyDeviceToDevice, s_main));
+    wAL[0][site] = cL;
+    wAR[0][site] = nk;
+  } else {
+    const int nk = bond[site + 1], cR = bond[site + 2];
+    const T* Bnew = mps[site + 1];
+    if (wTr > 0 && wAR[1][site + 1] == wTr && wTc == cR) {
+      overlap_right_step<T>(s_main, wT[wCur], wTr, wTc, wA[1][site + 1], wAL[1][site + 1], Bnew,
+                            nk, d, wT[1 - wCur], wTmp);
+      wCur = 1 - wCur;
+      wTr = wAL[1][site + 1];
+      wTc = nk;
+    } else {
+      wTr = wTc = 0;
+    }
+    DMRG_HIP(hipMemcpyAsync(wA[1][site + 1], Bnew, (size_t)nk * d * cR * sizeof(T),
+                            hipMemcpyDeviceToDevice, s_main));
+    wAL[1][site + 1] = nk;
+    wAR[1][site + 1] = cR;
+  }
 }
 
 template <class T>
@@ -424,6 +543,7 @@ double Engine<T>::optimize_bond(int site, char dir, bool rq) {
 template <class T>
 double Engine<T>::sweep2_lr() {
   double E = 0.0;
+  warm_begin();
   for (int site = 0; site < L - 1; ++site) {
     E = optimize_bond(site, 'R', false);
     DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));
@@ -438,6 +558,7 @@ double Engine<T>::sweep2_lr() {
 template <class T>
 double Engine<T>::sweep2_rl() {
   double E = 0.0;
+  warm_begin();
   for (int site = L - 2; site >= 0; --site) {
     E = optimize_bond(site, 'L', site == 0);
     DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));

@@ -10,6 +10,7 @@
 #include "contract.hpp"
 #include "lanczos.hpp"
 #include "svd.hpp"
+#include "warm.hpp"
 
 namespace dmrg {
 
@@ -24,6 +25,7 @@ struct Options {
   int device = 0;
   int svd_max_sweeps = 40;
   double svd_tol = 1e-14;
+  int svd_warm = 1;  // two-site: start each split's Jacobi from the previous sweep's basis
 };
 
 struct Stats {
@@ -33,6 +35,7 @@ struct Stats {
   int sweeps_done = 0;
   double max_discarded = 0;
   double env_build_s = 0, sweeps_s_total = 0;
+  long long svd_warm_starts = 0;
 };
 
 // Algorithmic flop counts (real arithmetic; complex = 4x).
@@ -111,6 +114,12 @@ class Engine {
   double lanczos_one(int site);
   void matvec2(int site, const T* in, T* out, bool fp64);
   void split_two(int site, char dir);
+  // warm start of the truncation SVD (two-site; see warm.hpp)
+  void warm_alloc();
+  void warm_free();
+  void warm_begin();
+  const T* warm_basis(int site, char dir);
+  void warm_after_split(int site, char dir);
   void split_one(int site, char dir);
   double rayleigh2(int site);
 
@@ -136,6 +145,15 @@ class Engine {
     return p;
   }
   LanczosRunner<T> runner;
+  bool warm_on = false;
+  std::vector<T*> wX[2];           // per bond: last sorted eigenbasis (N x N)
+  std::vector<int> wXN[2], wXB[2];  // its N and block bond dimension (0: none)
+  std::vector<T*> wA[2];           // per site: canonical tensor of the last sweep
+  std::vector<int> wAL[2], wAR[2];  // its bond dimensions (0: none)
+  T* wT[2] = {nullptr, nullptr};   // running block overlap (ping-pong)
+  T *wZ0 = nullptr, *wZ1 = nullptr, *wP = nullptr, *wTmp = nullptr, *wXw = nullptr;
+  double* wDev = nullptr;
+  int wTr = 0, wTc = 0, wCur = 0;
   int* h_status = nullptr;
   double* h_scal = nullptr;
   size_t nmax = 0;

@@ -51,7 +51,7 @@ template <> __device__ inline cdouble get<cdouble>(const double* pl, int idx, in
 // Set-up: symmetrise the N x N block of rho (lower := conj(upper), real
 // diagonal), zero the padding up to Np, V = I.
 template <class T>
-__global__ void k_prep(T* __restrict__ A, T* __restrict__ V, int N, int Np) {
+__global__ void k_prep(T* __restrict__ A, T* __restrict__ V, int N, int Np, int set_v) {
   const size_t tot = (size_t)Np * Np;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
@@ -59,7 +59,18 @@ __global__ void k_prep(T* __restrict__ A, T* __restrict__ V, int N, int Np) {
     if (r >= N || c >= N) A[e] = T(0);
     else if (r > c) A[e] = conj(A[c + (size_t)r * Np]);
     else if (r == c) A[e] = T(real(A[e]));
-    V[e] = (r == c) ? T(1) : T(0);
+    if (set_v) V[e] = (r == c) ? T(1) : T(0);
+  }
+}
+
+// V (Np x Np) = the caller's N x N basis X (ld N), identity on the padding.
+template <class T>
+__global__ void k_set_basis(T* __restrict__ V, const T* __restrict__ X, int N, int Np) {
+  const size_t tot = (size_t)Np * Np;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % Np), c = (int)(e / Np);
+    V[e] = (r < N && c < N) ? X[r + (size_t)c * N] : ((r == c) ? T(1) : T(0));
   }
 }
 
@@ -555,11 +566,12 @@ inline unsigned grid_for(size_t tot) { return (unsigned)std::min<size_t>(cdiv(to
 // w.A[cur], the eigenvectors in w.V[cur] and the descending order in w.perm.
 template <class T>
 int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, int& sweeps,
-           double& off_rel) {
+           double& off_rel, bool v_given = false) {
   const int nb = Np / JB;
   const int np = nb / 2;
   const size_t tot = (size_t)Np * Np;
-  hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[0], w.V[0], N, Np);
+  hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[0], w.V[0], N, Np,
+                     v_given ? 0 : 1);
   hipLaunchKernelGGL(k_norms<T>, dim3(1), dim3(1024), 0, st, w.A[0], N, Np, w.scal);
   DMRG_LAUNCH_CHECK();
   double hs[3] = {0, 0, 0};
@@ -662,7 +674,7 @@ void SvdWork<T>::release() {
 template <class T>
 SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int lda, int kmax,
                     double cutoff, double rel_cutoff, bool scale_left, T* left, int ldl, T* right,
-                    double* sigma_out, const SvdParams& prm) {
+                    double* sigma_out, const SvdParams& prm, const T* warm, T* basis_out) {
   DMRG_REQUIRE(m > 0 && n > 0 && lda >= m && kmax >= 1, "svd: bad shape");
   DMRG_REQUIRE(left && right && ldl >= m, "svd: bad output");
   const int N = scale_left ? n : m;
@@ -676,9 +688,20 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
   else
     gemm<T>(st, OP_H, OP_N, n, n, m, T(1), A, lda, A, lda, T(0), w.A[0], Np,
             gemm_pick_tile(n, n, m, is_cplx<T>::value));
+  if (warm) {
+    // start from the caller's orthonormal basis X: rho <- X^H rho X, V = X
+    hipLaunchKernelGGL(k_set_basis<T>, dim3(grid_for((size_t)Np * Np)), dim3(256), 0, st, w.V[0],
+                       warm, N, Np);
+    const GemmTile tl = gemm_pick_tile(N, N, N, is_cplx<T>::value);
+    gemm<T>(st, OP_N, OP_N, N, N, N, T(1), w.A[0], Np, w.V[0], Np, T(0), w.A[1], Np, tl);
+    gemm<T>(st, OP_H, OP_N, N, N, N, T(1), w.V[0], Np, w.A[1], Np, T(0), w.A[0], Np, tl);
+  }
   SvdResult res;
   res.N = N;
-  const int cur = jacobi<T>(st, w, N, Np, prm, res.sweeps, res.off);
+  const int cur = jacobi<T>(st, w, N, Np, prm, res.sweeps, res.off, warm != nullptr);
+  if (basis_out)
+    hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)N * N)), dim3(256), 0, st,
+                       (const T*)w.V[cur], Np, (const int*)w.perm, N, N, 0, basis_out, N);
   if (!scale_left) {
     // U_k -> left; S Vh = U_k^H theta -> Y (k x n); G = Y Y^H
     hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)m * k)), dim3(256), 0, st,
@@ -745,10 +768,10 @@ template struct SvdWork<double>;
 template struct SvdWork<cdouble>;
 template SvdResult svd_split<double>(hipStream_t, SvdWork<double>&, const double*, int, int, int,
                                      int, double, double, bool, double*, int, double*, double*,
-                                     const SvdParams&);
+                                     const SvdParams&, const double*, double*);
 template SvdResult svd_split<cdouble>(hipStream_t, SvdWork<cdouble>&, const cdouble*, int, int,
                                       int, int, double, double, bool, cdouble*, int, cdouble*,
-                                      double*, const SvdParams&);
+                                      double*, const SvdParams&, const cdouble*, cdouble*);
 template int heig<double>(hipStream_t, SvdWork<double>&, const double*, int, int, double*, double*,
                           int, const SvdParams&);
 template int heig<cdouble>(hipStream_t, SvdWork<cdouble>&, const cdouble*, int, int, double*,

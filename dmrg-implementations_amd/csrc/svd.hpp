@@ -74,11 +74,18 @@ struct SvdResult {
 //     leading dimension nk) = S Vh.
 //   scale_left == true ('V' side): left = U S, right = Vh.
 //   sigma_out (optional, device): the nk kept singular values.
+//   warm (optional, device, N x N, ld N; N = m for the 'U' side, n for 'V'):
+//     an orthonormal basis of that side to start the Jacobi from (rho is
+//     first rotated into it).  Any orthonormal basis gives the same result;
+//     one close to the eigenbasis needs fewer sweeps.
+//   basis_out (optional, device, N x N, ld N): receives the full sorted
+//     eigenbasis, the warm start of a later split at the same bond.
 // Synchronises the stream once per Jacobi sweep and once for the rank.
 template <class T>
 SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int lda, int kmax,
                     double cutoff, double rel_cutoff, bool scale_left, T* left, int ldl, T* right,
-                    double* sigma_out, const SvdParams& prm);
+                    double* sigma_out, const SvdParams& prm, const T* warm = nullptr,
+                    T* basis_out = nullptr);
 
 // Eigen-decomposition of a Hermitian matrix H (N x N, ldh): eigenvalues
 // descending into evals (device, N), eigenvectors into evecs (N x N, lde).

@@ -1,0 +1,134 @@
+// Warm start of the truncation SVD (see warm.hpp).
+#include "warm.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace dmrg {
+namespace {
+
+__device__ inline double absv(double a) { return fabs(a); }
+__device__ inline double absv(cdouble a) { return sqrt(a.x * a.x + a.y * a.y); }
+
+__global__ void k_set_one(double* dev) { *dev = 0.0; }
+
+// dev = max |P - I| (as an atomic max of the bit pattern); P <- (3I - P)/2.
+template <class T>
+__global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n,
+                                                 unsigned long long* __restrict__ dev) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double mx = 0.0;
+  const size_t tot = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + t; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    const T one = (r == c) ? T(1) : T(0);
+    const T p = P[e];
+    double v = absv(p - one);
+    if (!(v == v)) v = INFINITY;
+    mx = fmax(mx, v);
+    P[e] = (3.0 * one - p) * 0.5;
+  }
+  red[t] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmax(red[t], red[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    double v = red[0];
+    unsigned long long bits;
+    memcpy(&bits, &v, sizeof(v));
+    atomicMax(dev, bits);
+  }
+}
+
+}  // namespace
+
+template <class T>
+void overlap_left_step(hipStream_t st, const T* Tin, int r, int c, const T* Aold, int r2,
+                       const T* Anew, int c2, int d, T* Tout, T* tmp) {
+  const bool cx = is_cplx<T>::value;
+  for (int s = 0; s < d; ++s) {
+    // tmp (r x c2) = Tin (r x c) * Anew_s (c x c2, ld c*d)
+    gemm<T>(st, OP_N, OP_N, r, c2, c, T(1), Tin, r, Anew + (size_t)s * c, c * d, T(0), tmp, r,
+            gemm_pick_tile(r, c2, c, cx));
+    // Tout (r2 x c2) += Aold_s^H (r2 x r) * tmp
+    gemm<T>(st, OP_H, OP_N, r2, c2, r, T(1), Aold + (size_t)s * r, r * d, tmp, r,
+            s == 0 ? T(0) : T(1), Tout, r2, gemm_pick_tile(r2, c2, r, cx));
+  }
+}
+
+template <class T>
+void overlap_right_step(hipStream_t st, const T* Tin, int r, int c, const T* Bold, int r2,
+                        const T* Bnew, int c2, int d, T* Tout, T* tmp) {
+  const bool cx = is_cplx<T>::value;
+  for (int s = 0; s < d; ++s) {
+    // tmp (r x c2) = Tin (r x c) * Bnew_s^T, Bnew_s (c2 x c) at Bnew + s*c2, ld c2*d
+    gemm<T>(st, OP_N, OP_T, r, c2, c, T(1), Tin, r, Bnew + (size_t)s * c2, c2 * d, T(0), tmp, r,
+            gemm_pick_tile(r, c2, c, cx));
+    // Tout (r2 x c2) += conj(Bold_s) (r2 x r, at Bold + s*r2, ld r2*d) * tmp
+    gemm<T>(st, OP_C, OP_N, r2, c2, r, T(1), Bold + (size_t)s * r2, r2 * d, tmp, r,
+            s == 0 ? T(0) : T(1), Tout, r2, gemm_pick_tile(r2, c2, r, cx));
+  }
+}
+
+template <class T>
+bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
+                   double start_tol, T** Zout) {
+  const bool cx = is_cplx<T>::value;
+  DMRG_HIP(hipMemcpyAsync(Z0, Tin, (size_t)n * n * sizeof(T), hipMemcpyDeviceToDevice, st));
+  T* Z = Z0;
+  T* Zn = Z1;
+  auto* devb = reinterpret_cast<unsigned long long*>(dev);
+  const unsigned grid = (unsigned)std::min<size_t>(cdiv((size_t)n * n, 256), 1024);
+  for (int it = 0; it < 12; ++it) {
+    gemm<T>(st, OP_H, OP_N, n, n, n, T(1), Z, n, Z, n, T(0), P, n, gemm_pick_tile(n, n, n, cx));
+    hipLaunchKernelGGL(k_set_one, dim3(1), dim3(1), 0, st, dev);
+    hipLaunchKernelGGL(k_ns_prep<T>, dim3(grid), dim3(256), 0, st, P, n, devb);
+    DMRG_LAUNCH_CHECK();
+    double h = 0.0;
+    DMRG_HIP(hipMemcpyAsync(&h, dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    DMRG_HIP(hipStreamSynchronize(st));
+    if (!(h == h) || (it == 0 && h > start_tol)) return false;
+    if (h <= 1e-14) {
+      *Zout = Z;
+      return true;
+    }
+    gemm<T>(st, OP_N, OP_N, n, n, n, T(1), Z, n, P, n, T(0), Zn, n, gemm_pick_tile(n, n, n, cx));
+    std::swap(Z, Zn);
+  }
+  return false;
+}
+
+template <class T>
+void warm_basis_left(hipStream_t st, const T* Z, int cL, int d, const T* X, T* out) {
+  const int N = cL * d;
+  for (int s = 0; s < d; ++s)
+    gemm<T>(st, OP_H, OP_N, cL, N, cL, T(1), Z, cL, X + (size_t)s * cL, N, T(0),
+            out + (size_t)s * cL, N, gemm_pick_tile(cL, N, cL, is_cplx<T>::value));
+}
+
+template <class T>
+void warm_basis_right(hipStream_t st, const T* Z, int cR, int d, const T* X, T* out) {
+  const int N = d * cR;
+  // rows i = (s, j) -> s + N*j (two-level, n1 = d); k = b' -> d*b'; n = b -> d*b
+  const View2 vx{{d, 1, N}, {1 << 30, d, 0}};
+  const View2 vz{{1 << 30, 1, 0}, {1 << 30, cR, 0}};  // Z(b', b) at b' + cR b
+  gemm_tensor<T>(st, d * N, cR, cR, T(1), X, vx, false, false, Z, vz, true, true, T(0), out, vx,
+                 gemm_pick_tile(d * N, cR, cR, is_cplx<T>::value));
+}
+
+#define DMRG_WARM_INST(T)                                                                       \
+  template void overlap_left_step<T>(hipStream_t, const T*, int, int, const T*, int, const T*, \
+                                     int, int, T*, T*);                                         \
+  template void overlap_right_step<T>(hipStream_t, const T*, int, int, const T*, int, const T*, \
+                                      int, int, T*, T*);                                        \
+  template bool polar_unitary<T>(hipStream_t, const T*, int, T*, T*, T*, double*, double, T**);  \
+  template void warm_basis_left<T>(hipStream_t, const T*, int, int, const T*, T*);               \
+  template void warm_basis_right<T>(hipStream_t, const T*, int, int, const T*, T*);
+DMRG_WARM_INST(double)
+DMRG_WARM_INST(cdouble)
+#undef DMRG_WARM_INST
+
+}  // namespace dmrg

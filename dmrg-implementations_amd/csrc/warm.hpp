@@ -1,0 +1,54 @@
+// Warm start of the truncation SVD from the previous sweep (two-site engine).
+//
+// At bond i of a left-to-right sweep the density matrix rho = theta theta^H
+// lives in the basis (a, s) of the left block (sites 0..i-1, bond index a)
+// times site i.  The previous left-to-right sweep diagonalised rho at the same
+// bond in the basis of the left block as it was then.  The overlap
+// T[a_old, a_new] = <a_old|a_new> of the two left blocks follows from the
+// saved left-canonical tensors by the transfer recursion
+//     T_j = sum_s A_old[j]_s^H T_{j-1} A_new[j]_s,   T_{-1} = [1],
+// and (P^H (x) I_d) X_old, with P the unitary polar factor of T, is an exact
+// orthonormal basis of the new (a, s) space that is close to the new
+// eigenbasis once the sweep has converged.  Right-to-left sweeps mirror this
+// with the right blocks, T_j = sum_s conj(B_old[j]_s) T_{j+1} B_new[j]_s^T.
+// The warm basis only changes how many Jacobi sweeps the split needs; the
+// split itself is exact for any orthonormal start.
+#pragma once
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace dmrg {
+
+// Tout (r2 x c2) = sum_s Aold_s^H Tin Anew_s; Tin r x c, Aold (r, d, r2),
+// Anew (c, d, c2); tmp holds r * c2.
+template <class T>
+void overlap_left_step(hipStream_t st, const T* Tin, int r, int c, const T* Aold, int r2,
+                       const T* Anew, int c2, int d, T* Tout, T* tmp);
+
+// Tout (r2 x c2) = sum_s conj(Bold_s) Tin Bnew_s^T; Tin r x c, Bold (r2, d, r),
+// Bnew (c2, d, c); tmp holds r * c2.
+template <class T>
+void overlap_right_step(hipStream_t st, const T* Tin, int r, int c, const T* Bold, int r2,
+                        const T* Bnew, int c2, int d, T* Tout, T* tmp);
+
+// Unitary polar factor of the n x n matrix Tin by Newton-Schulz iteration
+// Z <- Z (3I - Z^H Z)/2.  Returns false (and leaves *Zout undefined) when
+// max|T^H T - I| > start_tol (too far from unitary to be worth a warm start)
+// or when the iteration does not reach max|Z^H Z - I| <= 1e-14.  Z0, Z1, P
+// are n x n work buffers; dev is one device double.  Synchronises the stream
+// once per iteration.
+template <class T>
+bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
+                   double start_tol, T** Zout);
+
+// Left side: out[(a + cL s), j] = sum_a' conj(Z[a', a]) X[(a' + cL s), j],
+// N = cL d columns (ld N).
+template <class T>
+void warm_basis_left(hipStream_t st, const T* Z, int cL, int d, const T* X, T* out);
+
+// Right side: out[(s + d b), j] = sum_b' conj(Z[b', b]) X[(s + d b'), j],
+// N = d cR columns (ld N).
+template <class T>
+void warm_basis_right(hipStream_t st, const T* Z, int cR, int d, const T* X, T* out);
+
+}  // namespace dmrg

@@ -50,7 +50,8 @@ class Options(C.Structure):
     _fields_ = [("krylov_max", C.c_int), ("lanczos_tol", C.c_double), ("eig_tol", C.c_double),
                 ("svd_cutoff", C.c_double), ("svd_rel_cutoff", C.c_double),
                 ("precision", C.c_int), ("profile", C.c_int), ("device", C.c_int),
-                ("svd_max_sweeps", C.c_int), ("svd_tol", C.c_double)]
+                ("svd_max_sweeps", C.c_int), ("svd_tol", C.c_double),
+                ("svd_warm", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -60,7 +61,7 @@ class Stats(C.Structure):
                 ("lanczos_calls", C.c_longlong), ("svd_calls", C.c_longlong),
                 ("svd_sweeps", C.c_longlong), ("sweeps_done", C.c_int),
                 ("max_discarded", C.c_double), ("env_build_s", C.c_double),
-                ("sweeps_s_total", C.c_double)]
+                ("sweeps_s_total", C.c_double), ("svd_warm_starts", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

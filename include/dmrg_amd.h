@@ -59,6 +59,8 @@ typedef struct dmrg_options {
   int device;            /* HIP device ordinal                                */
   int svd_max_sweeps;    /* Jacobi sweep cap of the SVD eigen-solver          */
   double svd_tol;        /* Jacobi stop: max |rho_ij| <= svd_tol * |rho|_F    */
+  int svd_warm;          /* two-site: start each split from the previous sweep's
+                            eigenbasis at that bond (1, default) or from I (0) */
 } dmrg_options;
 
 typedef struct dmrg_stats {
@@ -77,6 +79,7 @@ typedef struct dmrg_stats {
   double max_discarded;  /* largest truncated weight sum(sigma_i^2) of a bond */
   double env_build_s;    /* wall seconds of the last environment build      */
   double sweeps_s_total; /* wall seconds of all sweeps (environment build excluded) */
+  long long svd_warm_starts; /* splits that started from a previous sweep's basis */
 } dmrg_stats;
 
 typedef struct dmrg_engine* dmrg_handle;

@@ -144,3 +144,31 @@ def test_config1_single_site_L12(dmrg_amd):
     assert np.max(np.abs(np.array(sw) - np.array(sw_o))) < 1e-10 * L
     E_ed = -5.142090632840533
     assert E >= E_ed - 1e-12 and E - E_ed < 1e-6
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_svd_warm_start_matches_cold(dmrg_amd, cplx):
+    """Warm-starting each split's Jacobi from the previous sweep's eigenbasis
+    (mapped through the block overlap) changes only the iteration count: the
+    sweep energies equal the cold-start run's."""
+    L, chi = 12, 24
+    if cplx:
+        Ws = models.josephson_mpo(L, n_max=1, E_J=1.0, E_C=0.5, phi_ext=np.pi / 4)
+        d, D, dt = 3, 4, "c128"
+    else:
+        Ws = models.heisenberg_mpo(L)
+        d, D, dt = 2, 5, "f64"
+    out = {}
+    for warm in (0, 1):
+        eng = dmrg_amd.DMRG(L, d, chi, D, dtype=dt, tol=0.0, krylov_max=30, svd_warm=warm)
+        eng.set_mpo(Ws)
+        eng.initialize_mps_random(seed=31, scale=0.1)
+        eng.run(6)
+        out[warm] = (np.array(eng.sweep_energies()), eng.stats())
+        e, n2 = eng.expectation()
+        assert abs(n2 - 1) < 1e-10
+        eng.close()
+    assert out[1][1]["svd_warm_starts"] > 0 and out[0][1]["svd_warm_starts"] == 0
+    assert np.max(np.abs(out[1][0] - out[0][0])) < 1e-10 * L
+    # converged sweeps need fewer Jacobi sweeps per split when warm-started
+    assert out[1][1]["svd_sweeps"] < out[0][1]["svd_sweeps"]
