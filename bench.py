@@ -10,8 +10,10 @@ dimensions pre-sized to min(2^min(i,L-i), chi_max) as the reference does.
 A step is one full sweep (left-to-right plus right-to-left).  For each of the
 FP64 path and the mixed path (FP32 MFMA H_eff matvecs; FP64 environments,
 Lanczos and SVD) the engine runs W warm-up sweeps from the seeded state, then
-exactly K timed sweeps (the environment build is outside the timed region,
-as in the reference).  `value` is the mixed path's seconds per sweep.  The
+exactly K timed sweeps, one per engine call (the environment build before each
+call is outside the sweep timer, as in the reference).  `value` is the mixed
+path's mean seconds per sweep; the per-sweep times and their median are in
+`modes`.  The
 roofline entry is the H_eff matvec (three MFMA GEMM launches), timed with
 HIP events on the engine's stream.  |dE| compares the two paths after the
 same number of sweeps from the same state.
@@ -72,8 +74,13 @@ def run_gpu(dmrg_amd, wl, precision, steps, warmup):
         eng.run(warmup)
     eng.reset_stats()
     dmrg_amd.synchronize()
+    # one sweep per call: each call rebuilds the environments (outside the
+    # sweep timer, as in the reference) and times exactly one full sweep
+    per_sweep = []
     t0 = time.perf_counter()
-    E = eng.run(steps)
+    for _ in range(steps):
+        E = eng.run(1)
+        per_sweep.append(eng.stats()["sweep_s_last"])
     dmrg_amd.synchronize()
     wall = time.perf_counter() - t0
     st = eng.stats()
@@ -83,6 +90,8 @@ def run_gpu(dmrg_amd, wl, precision, steps, warmup):
                heff_tflops=heff_tf, heff_ms=st["heff_ms"], heff_flop=st["heff_flop"],
                matvecs=st["matvecs"], svd_ms=st["svd_ms"], lanczos_ms=st["lanczos_ms"],
                svd_calls=st["svd_calls"], svd_sweeps=st["svd_sweeps"],
+               svd_warm_starts=st["svd_warm_starts"], sweep_s_each=per_sweep,
+               sweep_s_median=sorted(per_sweep)[len(per_sweep) // 2] if per_sweep else 0.0,
                ms_per_matvec=st["heff_ms"] / max(st["matvecs"], 1))
     return eng, res
 
@@ -115,8 +124,8 @@ def cpu_baseline(eng, wl, nbonds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-bonds", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")

@@ -1,6 +1,8 @@
 // Contractions of the sweep as MFMA GEMMs (see contract.hpp).
 #include "contract.hpp"
 
+#include <cstring>
+
 namespace dmrg {
 
 namespace {
@@ -129,6 +131,40 @@ __global__ void k_f32_f64(const float* in, double* out, size_t n) {
        i += (size_t)gridDim.x * blockDim.x)
     out[i] = (double)in[i];
 }
+__device__ inline double unit_scale(double s) { return s > 0.0 ? s : 1.0; }
+__global__ void k_zero_d(double* x) { *x = 0.0; }
+__global__ __launch_bounds__(256) void k_absmax(const double* in, size_t n, unsigned long long* out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double m = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + t; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(in[i]));
+  red[t] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmax(red[t], red[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    unsigned long long bits;
+    double v = red[0];
+    memcpy(&bits, &v, sizeof(v));
+    atomicMax(out, bits);
+  }
+}
+__global__ void k_f64_f32_scaled(const double* in, float* out, size_t n, const double* s) {
+  const double inv = 1.0 / unit_scale(*s);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)(in[i] * inv);
+}
+__global__ void k_f32_f64_scaled(const float* in, double* out, size_t n, const double* s1,
+                                 const double* s2) {
+  const double f = unit_scale(*s1) * unit_scale(*s2);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (double)in[i] * f;
+}
 template <class T>
 __global__ void k_unit(T* env, int D, int w) {
   int i = threadIdx.x;
@@ -146,6 +182,27 @@ void convert_f32_f64(hipStream_t st, const float* in, double* out, size_t n) {
   if (!n) return;
   hipLaunchKernelGGL(k_f32_f64, dim3(std::min<size_t>(cdiv(n, 256), 4096)), dim3(256), 0, st, in,
                      out, n);
+  DMRG_LAUNCH_CHECK();
+}
+void absmax_f64(hipStream_t st, const double* in, size_t n, double* out) {
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, st, out);
+  if (n)
+    hipLaunchKernelGGL(k_absmax, dim3(std::min<size_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, in,
+                       n, reinterpret_cast<unsigned long long*>(out));
+  DMRG_LAUNCH_CHECK();
+}
+void convert_f64_f32_scaled(hipStream_t st, const double* in, float* out, size_t n,
+                            const double* scale) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_f64_f32_scaled, dim3(std::min<size_t>(cdiv(n, 256), 4096)), dim3(256), 0,
+                     st, in, out, n, scale);
+  DMRG_LAUNCH_CHECK();
+}
+void convert_f32_f64_scaled(hipStream_t st, const float* in, double* out, size_t n,
+                            const double* s1, const double* s2) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_f32_f64_scaled, dim3(std::min<size_t>(cdiv(n, 256), 4096)), dim3(256), 0,
+                     st, in, out, n, s1, s2);
   DMRG_LAUNCH_CHECK();
 }
 template <class T>

@@ -55,6 +55,17 @@ void build_wright(const T* W, int d, int D, T* Wr);
 // Elementwise helpers.
 void convert_f64_f32(hipStream_t st, const double* in, float* out, size_t n);
 void convert_f32_f64(hipStream_t st, const float* in, double* out, size_t n);
+// out[0] = max |in[i]| (device scalar).
+void absmax_f64(hipStream_t st, const double* in, size_t n, double* out);
+// out = in / s (s = *scale, or 1 when *scale == 0).  The FP32 matvec keeps its
+// environments at unit scale: those of an unnormalised start state can exceed
+// the FP32 range (the right environments of a random L=128, chi=512 MPS reach
+// ~1e67).
+void convert_f64_f32_scaled(hipStream_t st, const double* in, float* out, size_t n,
+                            const double* scale);
+// out = in * s1 * s2 (each 1 when 0): undoes the two environment scales.
+void convert_f32_f64_scaled(hipStream_t st, const float* in, double* out, size_t n,
+                            const double* s1, const double* s2);
 template <class T>
 void set_boundary_env(hipStream_t st, T* env, int D, int w);  // (1, D, 1) unit vector e_w
 

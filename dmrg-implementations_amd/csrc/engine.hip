@@ -158,6 +158,7 @@ void Engine<T>::alloc_all() {
       WWf.assign(L - 1, nullptr);
       for (int i = 0; i < L - 1; ++i) dmalloc((void**)&WWf[i], K * K * sizeof(float));
       dmalloc((void**)&Lf, env_sz * sizeof(float));
+      dmalloc((void**)&envScale, 2 * sizeof(double));
       dmalloc((void**)&Rf, env_sz * sizeof(float));
       dmalloc((void**)&thf, nmax * sizeof(float));
       dmalloc((void**)&outf, nmax * sizeof(float));
@@ -203,7 +204,8 @@ void Engine<T>::free_all() {
   for (auto p : Wrd) f(p);
   for (auto p : WWd) f(p);
   for (auto p : WWf) f(p);
-  f(Lf); f(Rf); f(thf); f(outf);
+  f(Lf); f(Rf); f(thf); f(outf); f(envScale);
+  envScale = nullptr;
   f(sc_f.X); f(sc_f.Y);
   f(theta); f(work); f(V);
   f(sc_main.X); f(sc_main.Y); f(sc_env.X); f(sc_env.Y);
@@ -359,7 +361,7 @@ void Engine<T>::matvec2(int site, const T* in, T* out, bool fp64) {
       const size_t n = (size_t)cL * d * d * cR;
       convert_f64_f32(s_main, in, thf, n);
       heff2_apply<float>(s_main, Lf, Rf, WWf[site], thf, outf, cL, cR, d, D, sc_f);
-      convert_f32_f64(s_main, outf, out, n);
+      convert_f32_f64_scaled(s_main, outf, out, n, envScale, envScale + 1);
     }
   }
 }
@@ -370,8 +372,11 @@ double Engine<T>::lanczos_two(int site, bool rq) {
   const size_t n = (size_t)cL * d * d * cR;
   if (opt.precision == 1) {
     if constexpr (!is_cplx<T>::value) {
-      convert_f64_f32(s_main, Lenv[site], Lf, (size_t)cL * D * cL);
-      convert_f64_f32(s_main, Renv[site + 2], Rf, (size_t)cR * D * cR);
+      const size_t nl = (size_t)cL * D * cL, nr = (size_t)cR * D * cR;
+      absmax_f64(s_main, Lenv[site], nl, envScale);
+      absmax_f64(s_main, Renv[site + 2], nr, envScale + 1);
+      convert_f64_f32_scaled(s_main, Lenv[site], Lf, nl, envScale);
+      convert_f64_f32_scaled(s_main, Renv[site + 2], Rf, nr, envScale + 1);
     }
   }
   double t0 = now_s();

@@ -133,6 +133,7 @@ class Engine {
   std::vector<T*> Lenv, Renv, Wd, Wrd, WWd;
   std::vector<float*> WWf;
   float *Lf = nullptr, *Rf = nullptr, *thf = nullptr, *outf = nullptr;
+  double* envScale = nullptr;  // max |L|, max |R| of the current bond (FP32 mode)
   T *theta = nullptr, *work = nullptr, *V = nullptr;
   ContractScratch<T> sc_main, sc_env;
   ContractScratch<float> sc_f;

@@ -530,6 +530,58 @@ __global__ __launch_bounds__(256) void k_qr_sigma(const T* __restrict__ G, int l
   }
 }
 
+// Clean pivots of the QR step, normalised: vector i (row i of Y when by_row,
+// else column i) scaled by 1/sqrt(G_ii) when its pass-0 residual keeps at
+// least half of its norm (r0[i] >= G_ii / 2), zero otherwise.
+template <class T>
+__global__ void k_clean_scale(const T* __restrict__ Y, int ldy, int k, int len, int by_row,
+                              const T* __restrict__ G, int ldg, const double* __restrict__ r0,
+                              T* __restrict__ out, int ldo) {
+  const size_t tot = (size_t)k * len;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    int i, c;
+    if (by_row) { i = (int)(e % k); c = (int)(e / k); }
+    else { c = (int)(e % len); i = (int)(e / len); }
+    const double gii = real(G[i + (size_t)i * ldg]);
+    const bool clean = gii > 0.0 && r0[i] >= 0.5 * gii;
+    const double f = clean ? 1.0 / sqrt(gii) : 0.0;
+    if (by_row) out[i + (size_t)c * ldo] = Y[i + (size_t)c * ldy] * f;
+    else out[c + (size_t)i * ldo] = Y[c + (size_t)i * ldy] * f;
+  }
+}
+
+// Keep only the entries of C (k x k) that couple a target vector to an
+// earlier pivot: C[j, i], i < j (keep_lower) or C[i, j], i < j.
+template <class T>
+__global__ void k_mask_tri(T* __restrict__ C, int k, int keep_lower) {
+  const size_t tot = (size_t)k * k;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % k), c = (int)(e / k);
+    const bool keep = keep_lower ? (r > c) : (r < c);
+    if (!keep) C[e] = T(0);
+  }
+}
+
+// sigma[j] = |row j| (by_row: Y is k x len, ld ldy) or |column j|.
+template <class T>
+__global__ __launch_bounds__(256) void k_vecnorm(const T* __restrict__ Y, int ldy, int len,
+                                                 int by_row, double* __restrict__ sigma) {
+  __shared__ double red[256];
+  const int j = blockIdx.x, t = threadIdx.x;
+  double s = 0.0;
+  for (int i = t; i < len; i += 256)
+    s += abs2(by_row ? Y[j + (size_t)i * ldy] : Y[i + (size_t)j * ldy]);
+  red[t] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  if (t == 0) sigma[j] = sqrt(red[0]);
+}
+
 // Truncation rank (scalar_traits.h:497-505 semantics, plus the optional
 // relative cutoff) and discarded weight trace - sum_{j<nk} sigma_j^2.
 __global__ void k_rank(const double* __restrict__ sigma, int k, double cutoff, double rel,
@@ -719,10 +771,43 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
     gemm<T>(st, OP_H, OP_N, k, k, m, T(1), left, ldl, left, ldl, T(0), w.A[1 - cur], k,
             gemm_pick_tile(k, k, m, is_cplx<T>::value));
   }
+  // sigma_j: residual of product vector j after projecting out the earlier
+  // clean pivots, computed on the vectors themselves (absolute accuracy
+  // ~eps |theta|, like |R_jj| of a QR), pivots chosen from the Gram matrix.
   hipLaunchKernelGGL(k_qr_sigma<T>, dim3(k), dim3(256), 0, st, (const T*)w.A[1 - cur], k, 0,
                      (const double*)nullptr, w.resid);
-  hipLaunchKernelGGL(k_qr_sigma<T>, dim3(k), dim3(256), 0, st, (const T*)w.A[1 - cur], k, 1,
-                     (const double*)w.resid, w.sigma);
+  {
+    const bool byr = !scale_left;            // 'U' side: vectors are the rows of Y (k x n)
+    const T* Yv = byr ? w.Y : left;
+    const int ldy = byr ? k : ldl;
+    const int len = byr ? n : m;
+    T* Qc = w.V[1 - cur];
+    T* C = w.A[1 - cur];
+    T* R = w.A[cur];
+    const int ldq = byr ? k : m;
+    hipLaunchKernelGGL(k_clean_scale<T>, dim3(grid_for((size_t)k * len)), dim3(256), 0, st, Yv,
+                       ldy, k, len, byr ? 1 : 0, (const T*)w.A[1 - cur], k,
+                       (const double*)w.resid, Qc, ldq);
+    if (byr) {
+      gemm<T>(st, OP_N, OP_H, k, k, len, T(1), Yv, ldy, Qc, ldq, T(0), C, k,
+              gemm_pick_tile(k, k, len, is_cplx<T>::value));
+      hipLaunchKernelGGL(k_mask_tri<T>, dim3(grid_for((size_t)k * k)), dim3(256), 0, st, C, k, 1);
+      DMRG_HIP(hipMemcpy2DAsync(R, (size_t)k * sizeof(T), Yv, (size_t)ldy * sizeof(T),
+                                (size_t)k * sizeof(T), len, hipMemcpyDeviceToDevice, st));
+      gemm<T>(st, OP_N, OP_N, k, len, k, T(-1), C, k, Qc, ldq, T(1), R, k,
+              gemm_pick_tile(k, len, k, is_cplx<T>::value));
+      hipLaunchKernelGGL(k_vecnorm<T>, dim3(k), dim3(256), 0, st, (const T*)R, k, len, 1, w.sigma);
+    } else {
+      gemm<T>(st, OP_H, OP_N, k, k, len, T(1), Qc, ldq, Yv, ldy, T(0), C, k,
+              gemm_pick_tile(k, k, len, is_cplx<T>::value));
+      hipLaunchKernelGGL(k_mask_tri<T>, dim3(grid_for((size_t)k * k)), dim3(256), 0, st, C, k, 0);
+      DMRG_HIP(hipMemcpy2DAsync(R, (size_t)m * sizeof(T), Yv, (size_t)ldy * sizeof(T),
+                                (size_t)m * sizeof(T), k, hipMemcpyDeviceToDevice, st));
+      gemm<T>(st, OP_N, OP_N, len, k, k, T(-1), Qc, ldq, C, k, T(1), R, m,
+              gemm_pick_tile(len, k, k, is_cplx<T>::value));
+      hipLaunchKernelGGL(k_vecnorm<T>, dim3(k), dim3(256), 0, st, (const T*)R, m, len, 0, w.sigma);
+    }
+  }
   hipLaunchKernelGGL(k_rank, dim3(1), dim3(1), 0, st, (const double*)w.sigma, k, cutoff, rel_cutoff,
                      w.scal, w.newk);
   DMRG_LAUNCH_CHECK();
