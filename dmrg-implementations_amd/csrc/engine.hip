@@ -489,7 +489,10 @@ const T* Engine<T>::warm_basis(int site, char dir) {
   const int N = cb * d;
   if (wXN[wi][site] != N || wXB[wi][site] != cb || wTr != cb || wTc != cb) return nullptr;
   T* Z = nullptr;
-  if (!polar_unitary<T>(s_main, wT[wCur], cb, wZ0, wZ1, wP, wDev, 0.25, &Z)) return nullptr;
+  // any T with singular values in (0, 1] has a polar factor; the basis it
+  // gives is exact, and close to the new eigenbasis where the block changed
+  // little, so accept every convergent start
+  if (!polar_unitary<T>(s_main, wT[wCur], cb, wZ0, wZ1, wP, wDev, 1.01, &Z)) return nullptr;
   if (dir == 'R') warm_basis_left<T>(s_main, Z, cb, d, wX[wi][site], wXw);
   else warm_basis_right<T>(s_main, Z, cb, d, wX[wi][site], wXw);
   return wXw;

@@ -82,7 +82,7 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
   T* Zn = Z1;
   auto* devb = reinterpret_cast<unsigned long long*>(dev);
   const unsigned grid = (unsigned)std::min<size_t>(cdiv((size_t)n * n, 256), 1024);
-  for (int it = 0; it < 12; ++it) {
+  for (int it = 0; it < 40; ++it) {
     gemm<T>(st, OP_H, OP_N, n, n, n, T(1), Z, n, Z, n, T(0), P, n, gemm_pick_tile(n, n, n, cx));
     hipLaunchKernelGGL(k_set_one, dim3(1), dim3(1), 0, st, dev);
     hipLaunchKernelGGL(k_ns_prep<T>, dim3(grid), dim3(256), 0, st, P, n, devb);
@@ -90,7 +90,10 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
     double h = 0.0;
     DMRG_HIP(hipMemcpyAsync(&h, dev, sizeof(double), hipMemcpyDeviceToHost, st));
     DMRG_HIP(hipStreamSynchronize(st));
-    if (!(h == h) || (it == 0 && h > start_tol)) return false;
+    // singular values s of Z move as s <- s (3 - s^2)/2: from s << 1 about
+    // log_1.5(1/s) steps, then quadratically; a zero singular value never
+    // converges, so give up when the start is too far or progress stalls.
+    if (!(h == h) || (it == 0 && h > start_tol) || (it >= 20 && h > 0.5)) return false;
     if (h <= 1e-14) {
       *Zout = Z;
       return true;

@@ -33,8 +33,8 @@ void overlap_right_step(hipStream_t st, const T* Tin, int r, int c, const T* Bol
 
 // Unitary polar factor of the n x n matrix Tin by Newton-Schulz iteration
 // Z <- Z (3I - Z^H Z)/2.  Returns false (and leaves *Zout undefined) when
-// max|T^H T - I| > start_tol (too far from unitary to be worth a warm start)
-// or when the iteration does not reach max|Z^H Z - I| <= 1e-14.  Z0, Z1, P
+// max|T^H T - I| > start_tol, when the iteration stalls (a singular value
+// near 0) or when it does not reach max|Z^H Z - I| <= 1e-14 in 40 steps.  Z0, Z1, P
 // are n x n work buffers; dev is one device double.  Synchronises the stream
 // once per iteration.
 template <class T>
