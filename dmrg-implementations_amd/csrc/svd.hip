@@ -162,11 +162,13 @@ __global__ void k_permute(const T* __restrict__ A, const T* __restrict__ V, T* _
 // J^H [[a_aa, beta], [conj(beta), a_bb]] J = diag(a_aa - t|beta|, a_bb + t|beta|)
 // (the symmetric Schur decomposition with a phase, Golub & Van Loan 8.4).
 //
-// One wave per sub-problem: the 64 lanes synchronise with wave barriers only
-// (LDS operations of one wave complete in program order), so an inner round
-// costs no workgroup barrier.  The 2x2-block and Q updates are branch-free:
-// a pair that is not rotated carries (c, s) = (1, 0), an exact identity.
-constexpr int NTA = 64;
+//
+// NTA threads (4 waves) per sub-problem.  Each inner round: lanes < 16 compute
+// the round's 16 rotations (identity when |a_ab| <= thr), barrier, every
+// thread updates its 2x2 blocks and Q entries of rotated pairs, barrier.  A
+// sub-problem whose off-diagonal is already below 1e-6 |rho|_F gets one inner
+// sweep (the next outer sweep re-checks it), others up to inner_max.
+constexpr int NTA = 256;
 
 template <class T>
 __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int nb, int round,
@@ -177,32 +179,39 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
   __shared__ T Qs[JP * LDP];
   __shared__ double rc[JP / 2];
   __shared__ T rs[JP / 2];
+  __shared__ int rf[JP / 2];
   __shared__ int s_rot;
   const int k = blockIdx.x, t = threadIdx.x;
   int I, J;
   rr_pair(nb, round, k, I, J);
-  const double thr = rot_tol * scal[0];
+  const double fro = scal[0];
+  const double thr = rot_tol * fro;
   const double thr2 = thr * thr;
-  int big = 0;
+  const double near2 = 1e-12 * fro * fro;
+  int big = 0, far = 0;
   for (int e = t; e < JP * JP; e += NTA) {
     const int r = e % JP, c = e / JP;
     const T v = A[(size_t)gidx(I, J, r) + (size_t)gidx(I, J, c) * ld];
     As[c * LDP + r] = v;
     Qs[c * LDP + r] = (r == c) ? T(1) : T(0);
-    if (r != c && abs2(v) > thr2) big = 1;
+    if (r != c) {
+      const double a2v = abs2(v);
+      big |= a2v > thr2;
+      far |= a2v > near2;
+    }
   }
-  if (__ballot(big) == 0) {
+  if (t == 0) s_rot = 0;
+  if (!__syncthreads_or(big)) {
     if (t == 0) qflag[k] = 0;
     return;
   }
-  if (t == 0) s_rot = 0;
-  __syncwarp();
+  const int sweeps = __syncthreads_or(far) ? inner_max : 1;
   // Circle-schedule indices, advanced incrementally each round (index 0 is
   // fixed; the others step 1 -> 2 -> ... -> JP-1 -> 1), starting from
   // position == index.  After JP-1 rounds they are back at the start.
   auto adv = [](int i) { return i == 0 ? 0 : (i == JP - 1 ? 1 : i + 1); };
-  // HP = JP/2 pairs per round.  Lane t updates the 2x2 blocks of row pair
-  // k1 = t / TPR and column pairs k2 = t % TPR + TPR*q (TPR = NTA/HP lanes
+  // HP = JP/2 pairs per round.  Thread t updates the 2x2 blocks of row pair
+  // k1 = t / TPR and column pairs k2 = t % TPR + TPR*q (TPR = NTA/HP threads
   // per row pair), and the Q entries of row t % JP in pairs
   // kk = t / JP + (NTA/JP)*q.
   constexpr int HP = JP / 2;
@@ -220,7 +229,7 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
   int ap = t % HP, bp = JP - 1 - (t % HP);
   const int row = t % JP;
   int any = 0;
-  for (int sw = 0; sw < inner_max; ++sw) {
+  for (int sw = 0; sw < sweeps; ++sw) {
     for (int rd = 0; rd < JP - 1; ++rd) {
       if (t < HP) {
         const int a = ap, b = bp;
@@ -229,33 +238,47 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
         const double mag2 = abs2(be);
         double c = 1.0;
         T sp = T(0);
+        int f = 0;
         if (mag2 > thr2) {
-          // t = sign(tau)/(|tau| + sqrt(1+tau^2)) with tau = dl/(2|beta|), written
-          // as t = 2|beta| sgn w, w = 1/(|dl| + sqrt(dl^2 + 4|beta|^2)), so one
-          // sqrt, one reciprocal and one rsqrt give c, s e^{i phi} and t|beta|.
           const double dl = ga - al;
           const double sg = dl >= 0.0 ? 1.0 : -1.0;
-          const double w = 1.0 / (fabs(dl) + sqrt(dl * dl + 4.0 * mag2));
-          const double tb = 2.0 * sg * mag2 * w;   // t |beta|
-          c = rsqrt(1.0 + 4.0 * mag2 * w * w);     // 1/sqrt(1+t^2)
-          sp = be * (2.0 * sg * c * w);            // s e^{i phi} = t c beta/|beta|
+          double tb;
+          if (mag2 < 1e-18 * dl * dl) {
+            // |beta| << |dl|: t = beta/dl to O(t^3), c = 1 - t^2/2 to O(t^4),
+            // both exact to rounding here (t^2 < 1e-18)
+            const double inv = 1.0 / dl;
+            tb = mag2 * inv;
+            c = 1.0 - 0.5 * mag2 * inv * inv;
+            sp = be * (c * inv);
+          } else {
+            // t = sign(tau)/(|tau| + sqrt(1+tau^2)) with tau = dl/(2|beta|), written
+            // as t = 2|beta| sgn w, w = 1/(|dl| + sqrt(dl^2 + 4|beta|^2)), so one
+            // sqrt, one reciprocal and one rsqrt give c, s e^{i phi} and t|beta|.
+            const double w = 1.0 / (fabs(dl) + sqrt(dl * dl + 4.0 * mag2));
+            tb = 2.0 * sg * mag2 * w;                // t |beta|
+            c = rsqrt(1.0 + 4.0 * mag2 * w * w);     // 1/sqrt(1+t^2)
+            sp = be * (2.0 * sg * c * w);            // s e^{i phi} = t c beta/|beta|
+          }
           As[a * LDP + a] = T(al - tb);
           As[b * LDP + b] = T(ga + tb);
           As[b * LDP + a] = T(0);
           As[a * LDP + b] = T(0);
+          f = 1;
           s_rot = 1;
         }
         rc[t] = c;
         rs[t] = sp;
+        rf[t] = f;
       }
-      __syncwarp();
+      __syncthreads();
       // Off-diagonal 2x2 blocks: X <- J1^H X J2.
+      const int f1 = rf[k1];
       const double c1 = rc[k1];
       const T s1 = rs[k1];
 #pragma unroll
       for (int q = 0; q < NBQ; ++q) {
         const int k2 = t % TPR + TPR * q;
-        if (k1 == k2) continue;
+        if (k1 == k2 || !(f1 | rf[k2])) continue;
         const T x00 = As[a2[q] * LDP + a1], x01 = As[b2[q] * LDP + a1];
         const T x10 = As[a2[q] * LDP + b1], x11 = As[b2[q] * LDP + b1];
         const double c2 = rc[k2];
@@ -271,6 +294,7 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
 #pragma unroll
       for (int q = 0; q < NQQ; ++q) {
         const int kk = t / JP + (NTA / JP) * q;
+        if (!rf[kk]) continue;
         const T qa = Qs[aq[q] * LDP + row], qb = Qs[bq[q] * LDP + row];
         const double c = rc[kk];
         const T s = rs[kk];
@@ -285,12 +309,12 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
       for (int q = 0; q < NQQ; ++q) { aq[q] = adv(aq[q]); bq[q] = adv(bq[q]); }
       ap = adv(ap);
       bp = adv(bp);
-      __syncwarp();
+      __syncthreads();
     }
     const int rot = s_rot;
-    __syncwarp();
+    __syncthreads();
     if (t == 0) s_rot = 0;
-    __syncwarp();
+    __syncthreads();
     any |= rot;
     if (!rot) break;
   }
