@@ -54,6 +54,7 @@ Options to_opts(const dmrg_options* o) {
     r.svd_max_sweeps = o->svd_max_sweeps;
     r.svd_tol = o->svd_tol;
     r.svd_warm = o->svd_warm;
+    r.fp32_eig_rel = o->fp32_eig_rel;
   }
   return r;
 }
@@ -87,6 +88,7 @@ void dmrg_default_options(dmrg_options* o) {
   o->svd_max_sweeps = d.svd_max_sweeps;
   o->svd_tol = d.svd_tol;
   o->svd_warm = d.svd_warm;
+  o->fp32_eig_rel = d.fp32_eig_rel;
 }
 
 const char* dmrg_last_error(void) { return g_err.c_str(); }

@@ -48,7 +48,7 @@ double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int s
     lz_orth_norm<T>(st, *lz, work, V, n, it);
     if (it + 1 < max_iter) lz_scale<T>(st, *lz, V + (size_t)(it + 1) * n, work, n);
     if (it >= 4 && it % 3 == 0) {
-      lz_check(st, *lz, it + 1, beta_tol, eig_tol);
+      lz_check(st, *lz, it + 1, beta_tol, eig_tol, eig_rel);
       DMRG_HIP(hipMemcpyAsync(h_status, lz->status, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
       DMRG_HIP(hipStreamSynchronize(st));
       if (h_status[0]) {
@@ -189,6 +189,7 @@ void Engine<T>::alloc_all() {
   runner.h_scal = h_scal;
   runner.beta_tol = opt.lanczos_tol;
   runner.eig_tol = opt.eig_tol;
+  runner.eig_rel = opt.precision == 1 ? opt.fp32_eig_rel : 0.0;
 }
 
 template <class T>

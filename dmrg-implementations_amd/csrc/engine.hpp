@@ -26,6 +26,9 @@ struct Options {
   int svd_max_sweeps = 40;
   double svd_tol = 1e-14;
   int svd_warm = 1;  // two-site: start each split's Jacobi from the previous sweep's basis
+  // FP32 matvec mode: the Lanczos Ritz value cannot settle below the FP32
+  // rounding of H_eff, so the check also accepts |dE| < fp32_eig_rel |E|.
+  double fp32_eig_rel = 1e-9;
 };
 
 struct Stats {
@@ -63,7 +66,7 @@ struct LanczosRunner {
   int* h_status = nullptr;   // pinned [4]
   double* h_scal = nullptr;  // pinned [8]
   std::vector<hipEvent_t> ev;  // 2 per matvec
-  double beta_tol = 1e-12, eig_tol = 1e-12;
+  double beta_tol = 1e-12, eig_tol = 1e-12, eig_rel = 0.0;
   int last_niter = 0, last_matvecs = 0;
   double last_heff_ms = 0;
   double run(hipStream_t st, T* theta, size_t n, int kmax, int site,

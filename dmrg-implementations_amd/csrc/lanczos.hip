@@ -84,9 +84,17 @@ __global__ __launch_bounds__(NT) void k_norm(const T* __restrict__ x, size_t n, 
   __shared__ double sh[8];
   size_t b, e;
   chunk_range(n, b, e);
-  double acc = 0.0;
-  for (size_t i = b + threadIdx.x; i < e; i += NT) acc += abs2(x[i]);
-  acc = block_sum(acc, sh);
+  // four independent accumulators keep four loads in flight per thread
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  size_t i = b + threadIdx.x;
+  for (; i + 3 * NT < e; i += 4 * NT) {
+    a0 += abs2(x[i]);
+    a1 += abs2(x[i + NT]);
+    a2 += abs2(x[i + 2 * NT]);
+    a3 += abs2(x[i + 3 * NT]);
+  }
+  for (; i < e; i += NT) a0 += abs2(x[i]);
+  double acc = block_sum((a0 + a1) + (a2 + a3), sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
   if (!last_block(ticket)) return;
   __shared__ double tot;
@@ -139,9 +147,16 @@ __global__ __launch_bounds__(NT) void k_alpha(const T* __restrict__ v, const T* 
   __shared__ double sh[8];
   size_t b, e;
   chunk_range(n, b, e);
-  double acc = 0.0;
-  for (size_t i = b + threadIdx.x; i < e; i += NT) acc += real(conj(v[i]) * w[i]);
-  acc = block_sum(acc, sh);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  size_t i = b + threadIdx.x;
+  for (; i + 3 * NT < e; i += 4 * NT) {
+    a0 += real(conj(v[i]) * w[i]);
+    a1 += real(conj(v[i + NT]) * w[i + NT]);
+    a2 += real(conj(v[i + 2 * NT]) * w[i + 2 * NT]);
+    a3 += real(conj(v[i + 3 * NT]) * w[i + 3 * NT]);
+  }
+  for (; i < e; i += NT) a0 += real(conj(v[i]) * w[i]);
+  double acc = block_sum((a0 + a1) + (a2 + a3), sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
   if (!last_block(ticket)) return;
   __shared__ double tot;
@@ -165,6 +180,7 @@ __global__ __launch_bounds__(NT) void k_sub_dots(T* __restrict__ w, const T* __r
   const double bm = it > 0 ? beta[it - 1] : 0.0;
   const T* vi = V + (size_t)it * n;
   const T* vm = it > 0 ? V + (size_t)(it - 1) * n : nullptr;
+#pragma unroll 4
   for (size_t i = b + threadIdx.x; i < e; i += NT) {
     T x = w[i];
     x = x - a * vi[i];
@@ -174,12 +190,26 @@ __global__ __launch_bounds__(NT) void k_sub_dots(T* __restrict__ w, const T* __r
   const int nb = gridDim.x;
   for (int j0 = 0; j0 <= it; j0 += 4) {
     T acc[4] = {T(0), T(0), T(0), T(0)};
-    for (size_t i = b + threadIdx.x; i < e; i += NT) {
-      T x = w[i];
+    T acc2[4] = {T(0), T(0), T(0), T(0)};
+    size_t i = b + threadIdx.x;
+    for (; i + NT < e; i += 2 * NT) {
+      const T x = w[i], x2 = w[i + NT];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+        if (j0 + jj <= it) {
+          const T* vj = V + (size_t)(j0 + jj) * n;
+          acc[jj] += conj(vj[i]) * x;
+          acc2[jj] += conj(vj[i + NT]) * x2;
+        }
+    }
+    for (; i < e; i += NT) {
+      const T x = w[i];
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj)
         if (j0 + jj <= it) acc[jj] += conj(V[(size_t)(j0 + jj) * n + i]) * x;
     }
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) acc[jj] += acc2[jj];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
       if (j0 + jj > it) break;
@@ -220,14 +250,27 @@ __global__ __launch_bounds__(NT) void k_orth_norm(T* __restrict__ w, const T* __
   __syncthreads();
   size_t b, e;
   chunk_range(n, b, e);
-  double acc = 0.0;
-  for (size_t i = b + threadIdx.x; i < e; i += NT) {
+  double a0 = 0.0, a1 = 0.0;
+  size_t i = b + threadIdx.x;
+  for (; i + NT < e; i += 2 * NT) {
+    T x = w[i], x2 = w[i + NT];
+    for (int j = 0; j <= it; ++j) {
+      const T* vj = V + (size_t)j * n;
+      x = x - cs[j] * vj[i];
+      x2 = x2 - cs[j] * vj[i + NT];
+    }
+    w[i] = x;
+    w[i + NT] = x2;
+    a0 += abs2(x);
+    a1 += abs2(x2);
+  }
+  for (; i < e; i += NT) {
     T x = w[i];
     for (int j = 0; j <= it; ++j) x = x - cs[j] * V[(size_t)j * n + i];
     w[i] = x;
-    acc += abs2(x);
+    a0 += abs2(x);
   }
-  acc = block_sum(acc, sh);
+  double acc = block_sum(a0 + a1, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
   if (!last_block(ticket)) return;
   __shared__ double tot;
@@ -398,8 +441,8 @@ __device__ void tri_vector(const double* a, const double* b, int k, double lam, 
 }
 
 __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* beta, int k, int mode,
-                                           double beta_tol, double eig_tol, double* scal,
-                                           int* status, double* y) {
+                                           double beta_tol, double eig_tol, double eig_rel,
+                                           double* scal, int* status, double* y) {
   __shared__ double b2[128];
   __shared__ int cnt[64];
   __shared__ int bidx;
@@ -421,7 +464,7 @@ __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* b
     if (t == 0) {
       scal[4] = lam;
       double prev = scal[2];
-      if (fabs(lam - prev) < eig_tol) { status[0] = 1; status[1] = k; }
+      if (fabs(lam - prev) < fmax(eig_tol, eig_rel * fabs(lam))) { status[0] = 1; status[1] = k; }
       else { status[0] = 0; scal[2] = lam; }
     }
   } else {  // final
@@ -528,17 +571,18 @@ void lz_orth_norm(hipStream_t st, LzDev& d, T* w, const T* V, size_t n, int it) 
   DMRG_LAUNCH_CHECK();
 }
 
-void lz_check(hipStream_t st, LzDev& d, int ncheck, double beta_tol, double eig_tol) {
+void lz_check(hipStream_t st, LzDev& d, int ncheck, double beta_tol, double eig_tol,
+              double eig_rel) {
   DMRG_REQUIRE(ncheck <= 128, "lanczos: tridiagonal larger than 128");
   hipLaunchKernelGGL(k_tri, dim3(1), dim3(64), 0, st, d.alpha, d.beta, ncheck, 0, beta_tol, eig_tol,
-                     d.scal, d.status, d.y);
+                     eig_rel, d.scal, d.status, d.y);
   DMRG_LAUNCH_CHECK();
 }
 
 void lz_final(hipStream_t st, LzDev& d, int niter) {
   DMRG_REQUIRE(niter >= 1 && niter <= 128, "lanczos: bad Krylov dimension");
-  hipLaunchKernelGGL(k_tri, dim3(1), dim3(64), 0, st, d.alpha, d.beta, niter, 1, 0.0, 0.0, d.scal,
-                     d.status, d.y);
+  hipLaunchKernelGGL(k_tri, dim3(1), dim3(64), 0, st, d.alpha, d.beta, niter, 1, 0.0, 0.0, 0.0,
+                     d.scal, d.status, d.y);
   DMRG_LAUNCH_CHECK();
 }
 

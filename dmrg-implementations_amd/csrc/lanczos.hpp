@@ -46,7 +46,9 @@ template <class T> void lz_orth_norm(hipStream_t st, LzDev& d, T* w, const T* V,
 // prev_energy := 1e30, status := 0 (start of a Lanczos call).
 void lz_reset(hipStream_t st, LzDev& d);
 // Convergence check on T(ncheck) (every third iteration).
-void lz_check(hipStream_t st, LzDev& d, int ncheck, double beta_tol, double eig_tol);
+// Converged when |E - E_prev| < max(eig_tol, eig_rel |E|).
+void lz_check(hipStream_t st, LzDev& d, int ncheck, double beta_tol, double eig_tol,
+              double eig_rel = 0.0);
 // Lowest eigenpair of T(niter) -> scal[3], y.
 void lz_final(hipStream_t st, LzDev& d, int niter);
 // out = V y (niter columns), scal[0]=|out|, scal[1]=1/|out|

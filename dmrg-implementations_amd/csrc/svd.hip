@@ -634,6 +634,58 @@ __global__ void k_copy_rows(const T* __restrict__ Y, int ldy, int nk, int n, T* 
   }
 }
 
+// First-order eigenvector correction of a nearly diagonal Hermitian A
+// (Np x Np, real part of the diagonal = lambda): Q = I + E with
+// E_ij = A_ij / (lambda_j - lambda_i) for pairs the perturbation resolves
+// (|lambda_j - lambda_i| > sep |A_ij|), 0 otherwise (left to Jacobi).  E is
+// anti-Hermitian, so Q is unitary up to O(|E|^2).
+template <class T>
+__global__ void k_oa_q(const T* __restrict__ A, T* __restrict__ Q, int N, int Np, double sep) {
+  const size_t tot = (size_t)Np * Np;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % Np), c = (int)(e / Np);
+    T q = (r == c) ? T(1) : T(0);
+    if (r != c && r < N && c < N) {
+      const T a = A[e];
+      const double g = real(A[c + (size_t)c * Np]) - real(A[r + (size_t)r * Np]);
+      if (fabs(g) > sep * sqrt(abs2(a))) q = a * (1.0 / g);
+    }
+    Q[e] = q;
+  }
+}
+
+// dev = max |P - I| (atomic max of the bit pattern); P <- (3I - P)/2.
+template <class T>
+__global__ __launch_bounds__(256) void k_ns_step(T* __restrict__ P, int n,
+                                                 unsigned long long* __restrict__ dev) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double mx = 0.0;
+  const size_t tot = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + t; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    const T one = (r == c) ? T(1) : T(0);
+    const T p = P[e];
+    double v = absv(p - one);
+    if (!(v == v)) v = INFINITY;
+    mx = fmax(mx, v);
+    P[e] = (3.0 * one - p) * 0.5;
+  }
+  red[t] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmax(red[t], red[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    double v = red[0];
+    unsigned long long bits;
+    memcpy(&bits, &v, sizeof(v));
+    atomicMax(dev, bits);
+  }
+}
+
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 inline unsigned grid_for(size_t tot) { return (unsigned)std::min<size_t>(cdiv(tot, 256), 8192); }
 
@@ -667,8 +719,60 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
     DMRG_LAUNCH_CHECK();
     return cur;
   }
+  auto offmax = [&](const T* M) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
+    hipLaunchKernelGGL(k_offmax<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0, st,
+                       M, N, Np, w.off);
+    DMRG_LAUNCH_CHECK();
+    unsigned long long bits = 0;
+    DMRG_HIP(hipMemcpyAsync(&bits, w.off, sizeof(bits), hipMemcpyDeviceToHost, st));
+    DMRG_HIP(hipStreamSynchronize(st));
+    double o;
+    std::memcpy(&o, &bits, sizeof(o));
+    if (std::isinf(o) || !(o == o)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
+    return o;
+  };
   bool done = false;
-  for (int sw = 0; sw < prm.max_sweeps; ++sw) {
+  if (v_given) {
+    // Started from a basis near the eigenbasis: up to two first-order
+    // (all-pairs, GEMM-only) corrections before any Jacobi sweep.
+    const GemmTile tl = gemm_pick_tile(Np, Np, Np, is_cplx<T>::value);
+    for (int it = 0; it < 2 && !done; ++it) {
+      const double o = offmax(w.A[0]);
+      off_rel = o / fro;
+      if (o <= prm.tol * fro) { done = true; break; }
+      if (o > 1e-6 * fro) break;  // not perturbative: Jacobi
+      hipLaunchKernelGGL(k_oa_q<T>, dim3(grid_for(tot)), dim3(256), 0, st, (const T*)w.A[0], w.Y,
+                         N, Np, 1e4);
+      // make Q unitary: Newton-Schulz Q <- Q (3I - Q^H Q)/2 until max|Q^H Q - I|
+      // was below 1e-8 before the last step (the error then squares to < 1e-16)
+      bool unitary = false;
+      for (int ns = 0; ns < 4 && !unitary; ++ns) {
+        gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), w.Y, Np, w.Y, Np, T(0), w.A[1], Np, tl);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
+        hipLaunchKernelGGL(k_ns_step<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0,
+                           st, w.A[1], Np, w.off);
+        gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, w.A[1], Np, T(0), w.V[1], Np, tl);
+        std::swap(w.Y, w.V[1]);
+        unsigned long long db = 0;
+        DMRG_HIP(hipMemcpyAsync(&db, w.off, sizeof(db), hipMemcpyDeviceToHost, st));
+        DMRG_HIP(hipStreamSynchronize(st));
+        double dv;
+        std::memcpy(&dv, &db, sizeof(dv));
+        unitary = dv <= 1e-8;
+      }
+      if (!unitary) break;  // leave it to Jacobi (A, V unchanged)
+      std::swap(w.Y, w.V[1]);  // Q2 -> V[1]
+      gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.A[0], Np, w.V[1], Np, T(0), w.Y, Np, tl);  // A Q2
+      gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), w.V[1], Np, w.Y, Np, T(0), w.A[1], Np, tl);  // Q2^H A Q2
+      gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.V[0], Np, w.V[1], Np, T(0), w.Y, Np, tl);  // V Q2
+      std::swap(w.A[0], w.A[1]);
+      std::swap(w.V[0], w.Y);
+      hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[0], w.V[0], N, Np, 0);
+      DMRG_LAUNCH_CHECK();
+    }
+  }
+  for (int sw = 0; sw < prm.max_sweeps && !done; ++sw) {
     // sort the diagonal, permute rho and V into the other buffer
     hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N,
                        w.perm, nullptr);

@@ -61,6 +61,9 @@ typedef struct dmrg_options {
   double svd_tol;        /* Jacobi stop: max |rho_ij| <= svd_tol * |rho|_F    */
   int svd_warm;          /* two-site: start each split from the previous sweep's
                             eigenbasis at that bond (1, default) or from I (0) */
+  double fp32_eig_rel;   /* FP32 matvec mode only: Lanczos also converges when
+                            |dE| < fp32_eig_rel*|E| (FP32 rounding of H_eff
+                            bounds how far the Ritz value settles); 1e-9 */
 } dmrg_options;
 
 typedef struct dmrg_stats {
