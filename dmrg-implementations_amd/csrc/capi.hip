@@ -321,7 +321,7 @@ struct CtxT {
     if (cap > svd.cap) svd.alloc(cap);
   }
   void need_lz(int kmax, size_t n) {
-    size_t pb = (size_t)(kmax + 1) * 2 * lz_blocks(n) * sizeof(double);
+    size_t pb = lz_part_doubles(kmax, n) * sizeof(double);
     if (lz.kmax < kmax || lz.part_bytes < pb) lz.alloc(std::max(kmax, 128), std::max(pb, lz.part_bytes), sizeof(T));
   }
 };

@@ -14,6 +14,15 @@ template <class T> inline GemmTile pickT(int M, int N, int K) {
 }
 }  // namespace
 
+// Y = X W for the MPO-bond step (W column-major K x N).
+template <class T>
+void mpo_step(hipStream_t st, int M, int N, int K, const T* X, View2 vx, const T* W, T* Y,
+              View2 vy) {
+  if (gemm_rows_small<T>(st, M, N, K, X, vx, W, Y, vy)) return;
+  gemm_tensor<T>(st, M, N, K, T(1), X, vx, false, false, W, view_colmajor(K), false, true, T(0), Y,
+                 vy, pickT<T>(M, N, K));
+}
+
 template <class T>
 void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, const T* theta, T* out,
                  int cL, int cR, int d, int D, ContractScratch<T>& sc) {
@@ -26,10 +35,8 @@ void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, cons
           pickT<T>(D * cL, dd * cR, cL));
   // Y[a' + cL ss' + cL dd b + cL dd cR n] = sum_{w,ss} X * WW[(w,ss),(ss',n)]
   View2 vx{{cL, (long long)D, (long long)D * cL * dd}, {D, 1, (long long)D * cL}};
-  View2 vw = view_colmajor(D * dd);
   View2 vy{{cL, 1, (long long)cL * dd}, {dd, (long long)cL, (long long)cL * dd * cR}};
-  gemm_tensor<T>(st, cL * cR, dd * D, D * dd, one, sc.X, vx, false, false, WW, vw, false, true,
-                 zero, sc.Y, vy, pickT<T>(cL * cR, dd * D, D * dd));
+  mpo_step<T>(st, cL * cR, dd * D, D * dd, sc.X, vx, WW, sc.Y, vy);
   // out[(a' + cL ss'), b'] = sum_{(b, n)} Y * R[(b + cR n), b']
   gemm<T>(st, OP_N, OP_N, cL * dd, cR, D * cR, one, sc.Y, cL * dd, Renv, cR * D, zero, out,
           cL * dd, pickT<T>(cL * dd, cR, D * cR));
@@ -44,10 +51,8 @@ void heff1_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* W, const
   gemm<T>(st, OP_T, OP_N, D * cL, d * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
           pickT<T>(D * cL, d * cR, cL));
   View2 vx{{cL, (long long)D, (long long)D * cL * d}, {D, 1, (long long)D * cL}};
-  View2 vw = view_colmajor(D * d);
   View2 vy{{cL, 1, (long long)cL * d}, {d, (long long)cL, (long long)cL * d * cR}};
-  gemm_tensor<T>(st, cL * cR, d * D, D * d, one, sc.X, vx, false, false, W, vw, false, true, zero,
-                 sc.Y, vy, pickT<T>(cL * cR, d * D, D * d));
+  mpo_step<T>(st, cL * cR, d * D, D * d, sc.X, vx, W, sc.Y, vy);
   gemm<T>(st, OP_N, OP_N, cL * d, cR, D * cR, one, sc.Y, cL * d, Renv, cR * D, zero, out, cL * d,
           pickT<T>(cL * d, cR, D * cR));
 }
@@ -61,10 +66,8 @@ void env_left(hipStream_t st, const T* Lenv, const T* A, const T* W, T* Lnew, in
   gemm<T>(st, OP_T, OP_N, D * ci, d * co, ci, one, Lenv, ci, A, ci, zero, sc.X, D * ci,
           pickT<T>(D * ci, d * co, ci));
   View2 vx{{ci, (long long)D, (long long)D * ci * d}, {D, 1, (long long)D * ci}};
-  View2 vw = view_colmajor(D * d);
   View2 vy{{ci, 1, (long long)ci * d}, {d, (long long)ci, (long long)ci * d * co}};
-  gemm_tensor<T>(st, ci * co, d * D, D * d, one, sc.X, vx, false, false, W, vw, false, true, zero,
-                 sc.Y, vy, pickT<T>(ci * co, d * D, D * d));
+  mpo_step<T>(st, ci * co, d * D, D * d, sc.X, vx, W, sc.Y, vy);
   gemm<T>(st, OP_T, OP_C, co * D, co, ci * d, one, sc.Y, ci * d, A, ci * d, zero, Lnew, co * D,
           pickT<T>(co * D, co, ci * d));
 }
@@ -80,10 +83,8 @@ void env_right(hipStream_t st, const T* Renv, const T* A, const T* Wr, T* Rnew, 
           pickT<T>(co * d, D * ci, ci));
   // Y[a + co w + co D sp + co D d b'] = sum_{s, w'} X * Wr[(s, w'), (w, sp)]
   View2 vx{{co, 1, (long long)co * d * D}, {d, (long long)co, (long long)co * d}};
-  View2 vw = view_colmajor(d * D);
   View2 vy{{co, 1, (long long)co * D * d}, {D, (long long)co, (long long)co * D}};
-  gemm_tensor<T>(st, co * ci, D * d, d * D, one, sc.X, vx, false, false, Wr, vw, false, true, zero,
-                 sc.Y, vy, pickT<T>(co * ci, D * d, d * D));
+  mpo_step<T>(st, co * ci, D * d, d * D, sc.X, vx, Wr, sc.Y, vy);
   // Rnew[(a + co w), a'] = sum_{(sp, b')} Y * conj(A[a', sp, b'])
   gemm<T>(st, OP_N, OP_H, co * D, co, d * ci, one, sc.Y, co * D, A, co, zero, Rnew, co * D,
           pickT<T>(co * D, co, d * ci));

@@ -46,7 +46,7 @@ double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int s
     lz_alpha<T>(st, *lz, vi, work, n, it);
     lz_sub_dots<T>(st, *lz, work, V, n, it);
     lz_orth_norm<T>(st, *lz, work, V, n, it);
-    if (it + 1 < max_iter) lz_scale<T>(st, *lz, V + (size_t)(it + 1) * n, work, n);
+    lz_next<T>(st, *lz, it + 1 < max_iter ? V + (size_t)(it + 1) * n : nullptr, work, n, it);
     if (it >= 4 && it % 3 == 0) {
       lz_check(st, *lz, it + 1, beta_tol, eig_tol, eig_rel);
       DMRG_HIP(hipMemcpyAsync(h_status, lz->status, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -176,8 +176,7 @@ void Engine<T>::alloc_all() {
   sc_env.cap = (size_t)D * d * chi * chi;
   dmalloc((void**)&sc_env.X, sc_env.cap * sizeof(T));
   dmalloc((void**)&sc_env.Y, sc_env.cap * sizeof(T));
-  const size_t nb = (size_t)lz_blocks(nmax);
-  lz.alloc(std::max(kmax, 1), (size_t)(kmax + 1) * 2 * nb * sizeof(double), sizeof(T));
+  lz.alloc(std::max(kmax, 1), lz_part_doubles(std::max(kmax, 1), nmax) * sizeof(double), sizeof(T));
   svdw.alloc(chi * d);
   if (variant == 2 && opt.svd_warm) warm_alloc();
   DMRG_HIP(hipHostMalloc((void**)&h_status, 4 * sizeof(int), hipHostMallocDefault));
@@ -399,6 +398,7 @@ double Engine<T>::rayleigh2(int site) {
   const size_t n = (size_t)cL * d * d * cR;
   matvec2(site, theta, work, true);
   lz_alpha<T>(s_main, lz, theta, work, n, 0);
+  lz_alpha_value(s_main, lz, n, 0);
   double a = 0.0;
   DMRG_HIP(hipMemcpyAsync(&a, lz.alpha, sizeof(double), hipMemcpyDeviceToHost, s_main));
   DMRG_HIP(hipStreamSynchronize(s_main));

@@ -2,13 +2,15 @@
 //
 // Workgroup: 256 threads = 4 wave64s in a 2x2 grid.  Each wave owns a
 // (BM/2) x (BN/2) sub-tile made of 16x16 MFMA blocks.  Operand tiles of depth
-// BK=16 are staged through LDS (double buffered, one-element row padding),
-// with the global loads of the next tile issued before the MFMAs of the
-// current one.  The MFMA is issued with the B fragment in the "A" slot so
+// BK (16 or 32) are staged through LDS (double buffered, one-element row
+// padding), with the global loads of the next tile (PF=1) or of the next two
+// tiles (PF=2) issued before the MFMAs of the current one.  The MFMA is issued with the B fragment in the "A" slot so
 // that each lane's four accumulator registers land on four columns of C with
 // consecutive lanes on consecutive rows: epilogue stores of a column-major C
 // are 128-byte runs.
 #include "gemm.hpp"
+
+#include <algorithm>
 
 namespace dmrg {
 namespace {
@@ -55,9 +57,8 @@ struct AddrTensor {
   __device__ long long operator()(int r, int c) const { return dim(v.row, r) + dim(v.col, c); }
 };
 
-constexpr int BK = 16;
-
-template <class T, bool CA, bool CB, bool KFA, bool KFB, int BM, int BN, class AA, class AB, class AC>
+template <class T, bool CA, bool CB, bool KFA, bool KFB, int BM, int BN, int BK, int PF, class AA,
+          class AB, class AC>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A,
                                               AA aa, long long sA, const T* __restrict__ B, AB ab,
                                               long long sB, T beta, T* __restrict__ C, AC ac,
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
   B += blockIdx.z * sB;
   C += blockIdx.z * sC;
 
-  T ra[NA], rb[NB];
+  static_assert(PF == 1 || PF == 2, "prefetch depth");
+  T ra[PF][NA], rb[PF][NB];
 
   auto idxA = [&](int r, int& m, int& k) {
     if (!KFA) { m = t % BM; k = t / BM + r * (256 / BM); }
@@ -90,7 +92,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
     if (!KFB) { n = t % BN; k = t / BN + r * (256 / BN); }
     else      { k = t % BK; n = t / BK + r * (256 / BK); }
   };
-  auto load = [&](int k0) {
+  // register stage `s` <- operand tiles at depth k0
+  auto load = [&](int s, int k0) {
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
       int m, k;
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
       int gm = m0 + m, gk = k0 + k;
       T v = T(0);
       if (gm < M && gk < K) v = A[aa(gm, gk)];
-      ra[r] = CA ? conj_v(v) : v;
+      ra[s][r] = CA ? conj_v(v) : v;
     }
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
@@ -107,16 +110,17 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
       int gn = n0 + n, gk = k0 + k;
       T v = T(0);
       if (gn < N && gk < K) v = B[ab(gk, gn)];
-      rb[r] = CB ? conj_v(v) : v;
+      rb[s][r] = CB ? conj_v(v) : v;
     }
   };
-  auto store = [&](int buf) {
+  // LDS buffer `buf` <- register stage `s`
+  auto store = [&](int s, int buf) {
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
       int m, k;
       idxA(r, m, k);
       E e[P];
-      split<T>(ra[r], e);
+      split<T>(ra[s][r], e);
 #pragma unroll
       for (int p = 0; p < P; ++p) As[buf][p][k][m] = e[p];
     }
@@ -125,7 +129,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
       int n, k;
       idxB(r, n, k);
       E e[P];
-      split<T>(rb[r], e);
+      split<T>(rb[s][r], e);
 #pragma unroll
       for (int p = 0; p < P; ++p) Bs[buf][p][k][n] = e[p];
     }
@@ -140,13 +144,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
       for (int j = 0; j < FN; ++j) acc[p][i][j] = V4{0, 0, 0, 0};
 
   const int nk = (K + BK - 1) / BK;
-  load(0);
-  store(0);
-  __syncthreads();
   const int li = lane & 15, lk = lane >> 4;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       E af[P][FM], bf[P][FN];
@@ -171,8 +170,36 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
           }
         }
     }
-    if (kt + 1 < nk) store(buf ^ 1);
+  };
+  load(0, 0);
+  store(0, 0);
+  if constexpr (PF == 1) {
     __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int buf = kt & 1;
+      if (kt + 1 < nk) load(0, (kt + 1) * BK);
+      compute(buf);
+      if (kt + 1 < nk) store(0, buf ^ 1);
+      __syncthreads();
+    }
+  } else {
+    // Two register stages: the loads of tile kt+2 are in flight while tile
+    // kt is multiplied and tile kt+1 (loaded one step earlier) is staged.
+    // Stage and LDS buffer of tile kt are both kt & 1; the loop is unrolled
+    // by two so that stage indices are compile-time constants.
+    if (nk > 1) load(1, BK);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt += 2) {
+      if (kt + 2 < nk) load(0, (kt + 2) * BK);
+      compute(0);
+      if (kt + 1 < nk) store(1, 1);
+      __syncthreads();
+      if (kt + 1 >= nk) break;
+      if (kt + 3 < nk) load(1, (kt + 3) * BK);
+      compute(1);
+      if (kt + 2 < nk) store(0, 0);
+      __syncthreads();
+    }
   }
 
   // Epilogue: lane -> row li of C; register r -> column lk + 4r (f64) or
@@ -202,16 +229,19 @@ template <class T, bool CA, bool CB, bool KFA, bool KFB, class AA, class AB, cla
 void launch(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* A, AA aa,
             long long sA, const T* B, AB ab, long long sB, T beta, T* C, AC ac, long long sC,
             int batch) {
-#define DMRG_GEMM_LAUNCH(BM_, BN_)                                                          \
-  hipLaunchKernelGGL((k_gemm<T, CA, CB, KFA, KFB, BM_, BN_, AA, AB, AC>),                   \
+#define DMRG_GEMM_LAUNCH(BM_, BN_, BK_, PF_)                                                \
+  hipLaunchKernelGGL((k_gemm<T, CA, CB, KFA, KFB, BM_, BN_, BK_, PF_, AA, AB, AC>),         \
                      dim3(cdiv(M, BM_), cdiv(N, BN_), batch), dim3(256), 0, st, M, N, K,    \
                      alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC)
+  // 32x32 tiles serve grids that cannot fill the chip with 64x64 ones; a deep
+  // k-step and two register stages then hide the load latency that the low
+  // tile count per CU leaves exposed (real types; complex already issues four
+  // MFMAs per k-step).
   if (tl.bm == 32 && tl.bn == 32) {
-    DMRG_GEMM_LAUNCH(32, 32);
-  } else if (tl.bm == 128 && tl.bn == 64 && !is_cplx<T>::value) {
-    DMRG_GEMM_LAUNCH(128, 64);
+    if constexpr (is_cplx<T>::value) DMRG_GEMM_LAUNCH(32, 32, 16, 1);
+    else DMRG_GEMM_LAUNCH(32, 32, 32, 2);
   } else {
-    DMRG_GEMM_LAUNCH(64, 64);
+    DMRG_GEMM_LAUNCH(64, 64, 16, 1);
   }
 #undef DMRG_GEMM_LAUNCH
   DMRG_LAUNCH_CHECK();
@@ -268,13 +298,59 @@ void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T
 
 GemmTile gemm_pick_tile(int M, int N, int K, bool complex_) {
   (void)K;
-  long long t64 = (long long)cdiv(M, 64) * cdiv(N, 64);
-  if (t64 < 256) return {32, 32};
-  if (!complex_) {
-    long long t128 = (long long)cdiv(M, 128) * cdiv(N, 64);
-    if (t128 >= 1024) return {128, 64};
-  }
+  const long long t64 = (long long)cdiv(M, 64) * cdiv(N, 64);
+  // 64x64 once there are about four tiles per CU (256 CUs)
+  if (t64 < (complex_ ? 256 : 1024)) return {32, 32};
   return {64, 64};
+}
+
+namespace {
+// One thread per row r: C(r, :) = A(r, :) B with B (K x N, column-major),
+// K <= KM and N <= 32.  B sits in LDS zero-padded to KM rows, so the unrolled
+// k-loop needs no bounds; the operand column offsets are tabulated once per
+// workgroup and each thread does one div/mod pair per operand row.
+template <class T, int KM>
+__global__ __launch_bounds__(256) void k_rows_small(int M, int N, int K, const T* __restrict__ A,
+                                                    View2 va, const T* __restrict__ B,
+                                                    T* __restrict__ C, View2 vc) {
+  __shared__ T Bs[KM * 32];
+  __shared__ long long ca[KM], cc[32];
+  for (int e = threadIdx.x; e < KM * 32; e += 256) {
+    const int k = e % KM, j = e / KM;
+    Bs[e] = (k < K && j < N) ? B[k + (size_t)K * j] : T(0);
+  }
+  for (int k = threadIdx.x; k < KM; k += 256) ca[k] = k < K ? AddrTensor::dim(va.col, k) : 0;
+  for (int j = threadIdx.x; j < N; j += 256) cc[j] = AddrTensor::dim(vc.col, j);
+  __syncthreads();
+  for (int r = blockIdx.x * 256 + threadIdx.x; r < M; r += gridDim.x * 256) {
+    const long long ra = AddrTensor::dim(va.row, r), rc = AddrTensor::dim(vc.row, r);
+    T x[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) x[k] = k < K ? A[ra + ca[k]] : T(0);
+    for (int j = 0; j < N; ++j) {
+      T y = T(0);
+#pragma unroll
+      for (int k = 0; k < KM; ++k) y = y + x[k] * Bs[k + KM * j];
+      C[rc + cc[j]] = y;
+    }
+  }
+}
+}  // namespace
+
+template <class T>
+bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, const T* B, T* C,
+                     View2 vc) {
+  if (K > 32 || N > 32 || K < 1) return false;
+  if (M == 0 || N == 0) return true;
+  const unsigned grid = (unsigned)std::min<long long>(cdiv(M, 256), 8192);
+  if (K <= 16)
+    hipLaunchKernelGGL((k_rows_small<T, 16>), dim3(grid), dim3(256), 0, st, M, N, K, A, va, B, C, vc);
+  else if (K <= 24)
+    hipLaunchKernelGGL((k_rows_small<T, 24>), dim3(grid), dim3(256), 0, st, M, N, K, A, va, B, C, vc);
+  else
+    hipLaunchKernelGGL((k_rows_small<T, 32>), dim3(grid), dim3(256), 0, st, M, N, K, A, va, B, C, vc);
+  DMRG_LAUNCH_CHECK();
+  return true;
 }
 
 template <class T>
@@ -338,7 +414,9 @@ void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2
                                 long long, const T*, int, long long, T, T*, int, long long,    \
                                 int, GemmTile);                                                \
   template void gemm_tensor<T>(hipStream_t, int, int, int, T, const T*, View2, bool, bool,     \
-                               const T*, View2, bool, bool, T, T*, View2, GemmTile);
+                               const T*, View2, bool, bool, T, T*, View2, GemmTile);           \
+  template bool gemm_rows_small<T>(hipStream_t, int, int, int, const T*, View2, const T*, T*,  \
+                                   View2);
 DMRG_INST(double)
 DMRG_INST(cdouble)
 DMRG_INST(float)

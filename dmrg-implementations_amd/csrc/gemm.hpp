@@ -20,7 +20,7 @@
 namespace dmrg {
 
 struct GemmTile {
-  int bm, bn;  // workgroup tile: 32x32, 64x64 or 128x64 (real only)
+  int bm, bn;  // workgroup tile: 32x32 or 64x64
 };
 
 // Index map of one operand dimension: i -> (i % n1) * s1 + (i / n1) * s2.
@@ -53,6 +53,15 @@ template <class T>
 void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2 va, bool conjA,
                  bool a_kfast, const T* B, View2 vb, bool conjB, bool b_kfast, T beta, T* C,
                  View2 vc, GemmTile tile = {64, 64});
+
+// C (M x N) = A B for tiny inner and outer sizes (K <= 32, N <= 32): the
+// MPO-bond step of H_eff and of the environment updates, where a tiled GEMM
+// wastes most of its MFMA width and the work is one pass over A and C.  A and
+// C use two-level addressing, B is column-major K x N.  Returns false (and
+// does nothing) when the sizes are out of range.
+template <class T>
+bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, const T* B, T* C,
+                     View2 vc);
 
 // Pick a tile so that the grid covers the chip.
 GemmTile gemm_pick_tile(int M, int N, int K, bool complex_);

@@ -16,19 +16,6 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Sum over the workgroup; result valid in thread 0.
-__device__ inline double block_sum(double v, double* sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
-  return r;
-}
-
 template <class T> struct Acc;
 template <> struct Acc<double> {
   static constexpr int N = 1;
@@ -41,24 +28,6 @@ template <> struct Acc<cdouble> {
   __device__ static cdouble get(const double* o) { return cdouble(o[0], o[1]); }
 };
 
-// Last-workgroup ticket: true in exactly one workgroup, after all others have
-// published their partials.
-__device__ inline bool last_block(unsigned* ticket) {
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(ticket, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-__device__ inline void reset_ticket(unsigned* ticket) {
-  if (threadIdx.x == 0) atomicExch(ticket, 0u);
-}
-
 __device__ inline void chunk_range(size_t n, size_t& b, size_t& e) {
   size_t chunk = (n + gridDim.x - 1) / gridDim.x;
   chunk = (chunk + NT - 1) / NT * NT;
@@ -66,22 +35,65 @@ __device__ inline void chunk_range(size_t n, size_t& b, size_t& e) {
   e = b + chunk < n ? b + chunk : n;
 }
 
-// Sum `cnt` partial vectors of length nb (layout part[j*nb + blk]) into out[j]
-// (doubles, stride-1), one wave per j.
-__device__ inline void reduce_parts(const double* part, int nb, int cnt, double* out) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int j = w; j < cnt; j += nw) {
+// Reductions across workgroups carry no fence or atomic: each producing
+// kernel writes one partial per workgroup, and the kernel that consumes the
+// sum re-reduces the partials itself (every workgroup, same fixed order, so
+// all workgroups see bit-identical values).  The kernel boundary makes the
+// partials visible.  A device-scope release inside a kernel would write back
+// the L2 of every XCD once per wave.
+
+// Sum of part[0..nb) over the workgroup; the result is returned to every thread.
+__device__ inline double sum_parts(const double* __restrict__ part, int nb, double* sh) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += NT) s += part[b];
+  s = wave_sum(s);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) r += sh[i];
+  __syncthreads();  // sh is free again on return
+  return r;
+}
+
+// out[j] = sum_b part[j*nb + b] for j < cnt (one wave per j); ends with a barrier.
+__device__ inline void sum_parts_multi(const double* __restrict__ part, int nb, int cnt, double* out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = w; j < cnt; j += NT / 64) {
     double s = 0.0;
     for (int b = lane; b < nb; b += 64) s += part[(size_t)j * nb + b];
     s = wave_sum(s);
     if (lane == 0) out[j] = s;
   }
+  __syncthreads();
 }
 
+// Partial sums of G values per thread -> part[(j0 + jj) * nb + blockIdx.x], jj < cnt.
+template <int G>
+__device__ inline void write_parts(const double (&v)[G], int cnt, double* part, int j0, int nb,
+                                   double* sh /*[G * NT/64]*/) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int jj = 0; jj < G; ++jj) {
+    const double s = wave_sum(v[jj]);
+    if (lane == 0) sh[jj * (NT / 64) + w] = s;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < cnt) {
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) r += sh[threadIdx.x * (NT / 64) + i];
+    part[(size_t)(j0 + threadIdx.x) * nb + blockIdx.x] = r;
+  }
+  __syncthreads();
+}
+
+// partN[blk] = sum |x_i|^2 over the workgroup's chunk.
 template <class T>
-__global__ __launch_bounds__(NT) void k_norm(const T* __restrict__ x, size_t n, double* part,
-                                            unsigned* ticket, double* scal) {
-  __shared__ double sh[8];
+__global__ __launch_bounds__(NT) void k_norm(const T* __restrict__ x, size_t n,
+                                            double* __restrict__ partN) {
+  __shared__ double sh[NT / 64];
   size_t b, e;
   chunk_range(n, b, e);
   // four independent accumulators keep four loads in flight per thread
@@ -94,18 +106,8 @@ __global__ __launch_bounds__(NT) void k_norm(const T* __restrict__ x, size_t n, 
     a3 += abs2(x[i + 3 * NT]);
   }
   for (; i < e; i += NT) a0 += abs2(x[i]);
-  double acc = block_sum((a0 + a1) + (a2 + a3), sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-  if (!last_block(ticket)) return;
-  __shared__ double tot;
-  reduce_parts(part, gridDim.x, 1, &tot);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double nrm = sqrt(tot);
-    scal[0] = nrm;
-    scal[1] = (nrm > 1e-300) ? 1.0 / nrm : 0.0;
-  }
-  reset_ticket(ticket);
+  const double v[1] = {(a0 + a1) + (a2 + a3)};
+  write_parts<1>(v, 1, partN, 0, gridDim.x, sh);
 }
 
 __device__ inline double u_pm1(uint64_t seed, uint64_t k) {
@@ -123,28 +125,43 @@ template <> __device__ inline cdouble rand_entry<cdouble>(uint64_t seed, uint64_
   return cdouble(u_pm1(seed, 2 * idx) * scale, u_pm1(seed, 2 * idx + 1) * scale);
 }
 
+// Fill x from the stream; with partN given, only when sqrt(sum partN) < thr.
 template <class T>
-__global__ void k_fill_random(T* x, size_t n, uint64_t seed, uint64_t offset, double scale,
-                              const double* scal, double thr) {
-  if (scal && !(scal[0] < thr)) return;
+__global__ __launch_bounds__(NT) void k_fill_random(T* x, size_t n, uint64_t seed, uint64_t offset,
+                                                   double scale, const double* partN, int nbp,
+                                                   double thr) {
+  __shared__ double sh[NT / 64];
+  if (partN && !(sqrt(sum_parts(partN, nbp, sh)) < thr)) return;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     x[i] = rand_entry<T>(seed, offset + i, scale);
 }
 
+// nrm = sqrt(sum partN); scal[0] = nrm, scal[1] = 1/nrm (0 if nrm <= 1e-300),
+// *beta_out = nrm (if given); dst = src / nrm (if dst given).
 template <class T>
-__global__ void k_scale(T* dst, const T* src, size_t n, const double* scal) {
-  const double s = scal[1];
+__global__ __launch_bounds__(NT) void k_scale(T* dst, const T* src, size_t n,
+                                             const double* __restrict__ partN, int nbp,
+                                             double* scal, double* beta_out) {
+  __shared__ double sh[NT / 64];
+  const double nrm = sqrt(sum_parts(partN, nbp, sh));
+  const double s = (nrm > 1e-300) ? 1.0 / nrm : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[0] = nrm;
+    scal[1] = s;
+    if (beta_out) *beta_out = nrm;
+  }
+  if (!dst) return;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i] * s;
 }
 
+// partA[blk] = Re <v, w> over the workgroup's chunk.
 template <class T>
 __global__ __launch_bounds__(NT) void k_alpha(const T* __restrict__ v, const T* __restrict__ w,
-                                             size_t n, double* part, unsigned* ticket,
-                                             double* alpha, int it) {
-  __shared__ double sh[8];
+                                             size_t n, double* __restrict__ partA) {
+  __shared__ double sh[NT / 64];
   size_t b, e;
   chunk_range(n, b, e);
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -156,98 +173,82 @@ __global__ __launch_bounds__(NT) void k_alpha(const T* __restrict__ v, const T* 
     a3 += real(conj(v[i + 3 * NT]) * w[i + 3 * NT]);
   }
   for (; i < e; i += NT) a0 += real(conj(v[i]) * w[i]);
-  double acc = block_sum((a0 + a1) + (a2 + a3), sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-  if (!last_block(ticket)) return;
-  __shared__ double tot;
-  reduce_parts(part, gridDim.x, 1, &tot);
-  __syncthreads();
-  if (threadIdx.x == 0) alpha[it] = tot;
-  reset_ticket(ticket);
+  const double vv[1] = {(a0 + a1) + (a2 + a3)};
+  write_parts<1>(vv, 1, partA, 0, gridDim.x, sh);
 }
 
-// w -= alpha v_it (+ beta v_{it-1}); c_j = <v_j, w> for j <= it.
+// out[it] = sum partA (one workgroup).
+__global__ __launch_bounds__(NT) void k_alpha_value(const double* __restrict__ partA, int nbp,
+                                                    double* alpha, int it) {
+  __shared__ double sh[NT / 64];
+  const double a = sum_parts(partA, nbp, sh);
+  if (threadIdx.x == 0) alpha[it] = a;
+}
+
+// alpha[it] = sum partA;  w -= alpha v_it + beta[it-1] v_{it-1};
+// partC[(j*NC + c)*nb + blk] = component c of <v_j, w> over the chunk, j <= it.
+// Dots are taken GD vectors at a time; the first group rides on the update pass.
 template <class T>
 __global__ __launch_bounds__(NT) void k_sub_dots(T* __restrict__ w, const T* __restrict__ V,
-                                                size_t n, int it, const double* alpha,
-                                                const double* beta, double* part,
-                                                unsigned* ticket, T* cvec) {
+                                                size_t n, int it, const double* __restrict__ partA,
+                                                double* alpha, const double* beta,
+                                                double* __restrict__ partC) {
   constexpr int NC = Acc<T>::N;
-  __shared__ double sh[8];
+  constexpr int GD = 8;
+  __shared__ double sh[GD * NC * (NT / 64)];
+  const int nb = gridDim.x;
+  const double a = sum_parts(partA, nb, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) alpha[it] = a;
   size_t b, e;
   chunk_range(n, b, e);
-  const double a = alpha[it];
   const double bm = it > 0 ? beta[it - 1] : 0.0;
   const T* vi = V + (size_t)it * n;
-  const T* vm = it > 0 ? V + (size_t)(it - 1) * n : nullptr;
-#pragma unroll 4
-  for (size_t i = b + threadIdx.x; i < e; i += NT) {
-    T x = w[i];
-    x = x - a * vi[i];
-    if (it > 0) x = x - bm * vm[i];
-    w[i] = x;
-  }
-  const int nb = gridDim.x;
-  for (int j0 = 0; j0 <= it; j0 += 4) {
-    T acc[4] = {T(0), T(0), T(0), T(0)};
-    T acc2[4] = {T(0), T(0), T(0), T(0)};
-    size_t i = b + threadIdx.x;
-    for (; i + NT < e; i += 2 * NT) {
-      const T x = w[i], x2 = w[i + NT];
+  const T* vm = it > 0 ? V + (size_t)(it - 1) * n : vi;
+  for (int j0 = 0; j0 <= it; j0 += GD) {
+    const int cnt = min(GD, it + 1 - j0);
+    T acc[GD];
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj)
-        if (j0 + jj <= it) {
-          const T* vj = V + (size_t)(j0 + jj) * n;
-          acc[jj] += conj(vj[i]) * x;
-          acc2[jj] += conj(vj[i + NT]) * x2;
-        }
-    }
-    for (; i < e; i += NT) {
-      const T x = w[i];
-#pragma unroll
-      for (int jj = 0; jj < 4; ++jj)
-        if (j0 + jj <= it) acc[jj] += conj(V[(size_t)(j0 + jj) * n + i]) * x;
-    }
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) acc[jj] += acc2[jj];
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      if (j0 + jj > it) break;
-      double o[2];
-      Acc<T>::put(acc[jj], o);
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        double s = block_sum(o[c], sh);
-        if (threadIdx.x == 0) part[((size_t)(j0 + jj) * NC + c) * nb + blockIdx.x] = s;
+    for (int jj = 0; jj < GD; ++jj) acc[jj] = T(0);
+    for (size_t i = b + threadIdx.x; i < e; i += NT) {
+      T x = w[i];
+      if (j0 == 0) {
+        x = x - a * vi[i];
+        if (it > 0) x = x - bm * vm[i];
+        w[i] = x;
       }
+#pragma unroll
+      for (int jj = 0; jj < GD; ++jj)
+        if (jj < cnt) acc[jj] += conj(V[(size_t)(j0 + jj) * n + i]) * x;
     }
+    double o[GD * NC];
+#pragma unroll
+    for (int jj = 0; jj < GD; ++jj) Acc<T>::put(acc[jj], o + jj * NC);
+    write_parts<GD * NC>(o, cnt * NC, partC, j0 * NC, nb, sh);
   }
-  if (!last_block(ticket)) return;
-  __shared__ double red[2 * 128];
-  const int cnt = (it + 1) * NC;
-  for (int c0 = 0; c0 < cnt; c0 += 2 * 128) {
-    int cc = min(2 * 128, cnt - c0);
-    reduce_parts(part + (size_t)c0 * nb, nb, cc, red);
-    __syncthreads();
-    for (int j = threadIdx.x; j < cc; j += NT) {
-      int g = c0 + j;
-      if (NC == 1) reinterpret_cast<double*>(cvec)[g] = red[j];
-      else reinterpret_cast<double*>(cvec)[g] = red[j];  // (re, im) interleaved == cdouble layout
-    }
-    __syncthreads();
-  }
-  reset_ticket(ticket);
 }
 
-// w -= sum_j c_j v_j; beta[it] = |w|.
+// cvec[g] = sum_b partC[g*nb + b], g < cnt (doubles; one workgroup).
+__global__ __launch_bounds__(NT) void k_coeffs(const double* __restrict__ partC, int nb, int cnt,
+                                               double* __restrict__ cvec) {
+  sum_parts_multi(partC, nb, cnt, cvec);
+}
+
+// w -= sum_j c_j v_j (c from cvec, or re-reduced from partC when cvec is null);
+// partN[blk] = |w|^2 over the chunk.
 template <class T>
 __global__ __launch_bounds__(NT) void k_orth_norm(T* __restrict__ w, const T* __restrict__ V,
-                                                 size_t n, int it, const T* cvec, double* part,
-                                                 unsigned* ticket, double* beta, double* scal) {
-  __shared__ double sh[8];
-  __shared__ T cs[128];
-  for (int j = threadIdx.x; j <= it; j += NT) cs[j] = cvec[j];
-  __syncthreads();
+                                                 size_t n, int it, const double* __restrict__ partC,
+                                                 const T* cvec, double* __restrict__ partN) {
+  constexpr int NC = Acc<T>::N;
+  __shared__ double sh[NT / 64];
+  __shared__ double cs[128 * NC];
+  const int nb = gridDim.x;
+  if (cvec) {
+    for (int j = threadIdx.x; j <= it; j += NT) Acc<T>::put(cvec[j], cs + j * NC);
+    __syncthreads();
+  } else {
+    sum_parts_multi(partC, nb, (it + 1) * NC, cs);
+  }
   size_t b, e;
   chunk_range(n, b, e);
   double a0 = 0.0, a1 = 0.0;
@@ -255,9 +256,10 @@ __global__ __launch_bounds__(NT) void k_orth_norm(T* __restrict__ w, const T* __
   for (; i + NT < e; i += 2 * NT) {
     T x = w[i], x2 = w[i + NT];
     for (int j = 0; j <= it; ++j) {
+      const T c = Acc<T>::get(cs + j * NC);
       const T* vj = V + (size_t)j * n;
-      x = x - cs[j] * vj[i];
-      x2 = x2 - cs[j] * vj[i + NT];
+      x = x - c * vj[i];
+      x2 = x2 - c * vj[i + NT];
     }
     w[i] = x;
     w[i + NT] = x2;
@@ -266,30 +268,19 @@ __global__ __launch_bounds__(NT) void k_orth_norm(T* __restrict__ w, const T* __
   }
   for (; i < e; i += NT) {
     T x = w[i];
-    for (int j = 0; j <= it; ++j) x = x - cs[j] * V[(size_t)j * n + i];
+    for (int j = 0; j <= it; ++j) x = x - Acc<T>::get(cs + j * NC) * V[(size_t)j * n + i];
     w[i] = x;
     a0 += abs2(x);
   }
-  double acc = block_sum(a0 + a1, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-  if (!last_block(ticket)) return;
-  __shared__ double tot;
-  reduce_parts(part, gridDim.x, 1, &tot);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double bt = sqrt(tot);
-    beta[it] = bt;
-    scal[1] = (bt > 1e-300) ? 1.0 / bt : 0.0;
-  }
-  reset_ticket(ticket);
+  const double v[1] = {a0 + a1};
+  write_parts<1>(v, 1, partN, 0, nb, sh);
 }
 
-// out = sum_j y_j v_j; scal[0]=|out|, scal[1]=1/|out|
+// out = sum_j y_j v_j; partN[blk] = |out|^2 over the chunk.
 template <class T>
 __global__ __launch_bounds__(NT) void k_ritz(T* __restrict__ out, const T* __restrict__ V, size_t n,
-                                            int niter, const double* y, double* part,
-                                            unsigned* ticket, double* scal) {
-  __shared__ double sh[8];
+                                            int niter, const double* y, double* __restrict__ partN) {
+  __shared__ double sh[NT / 64];
   __shared__ double ys[128];
   for (int j = threadIdx.x; j < niter; j += NT) ys[j] = y[j];
   __syncthreads();
@@ -302,18 +293,8 @@ __global__ __launch_bounds__(NT) void k_ritz(T* __restrict__ out, const T* __res
     out[i] = x;
     acc += abs2(x);
   }
-  acc = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-  if (!last_block(ticket)) return;
-  __shared__ double tot;
-  reduce_parts(part, gridDim.x, 1, &tot);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double nrm = sqrt(tot);
-    scal[0] = nrm;
-    scal[1] = (nrm > 1e-300) ? 1.0 / nrm : 0.0;
-  }
-  reset_ticket(ticket);
+  const double v[1] = {acc};
+  write_parts<1>(v, 1, partN, 0, gridDim.x, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -491,6 +472,8 @@ int lz_blocks(size_t n) {
   return (int)nb;
 }
 
+size_t lz_part_doubles(int kmax, size_t n) { return (size_t)(2 * kmax + 2) * lz_blocks(n); }
+
 void LzDev::alloc(int kmax_, size_t part_bytes_, size_t elem_bytes) {
   release();
   kmax = kmax_;
@@ -501,34 +484,44 @@ void LzDev::alloc(int kmax_, size_t part_bytes_, size_t elem_bytes) {
   DMRG_HIP(hipMalloc(&scal, 8 * sizeof(double)));
   DMRG_HIP(hipMalloc(&status, 4 * sizeof(int)));
   DMRG_HIP(hipMalloc(&part, part_bytes));
-  DMRG_HIP(hipMalloc(&ticket, 4 * sizeof(unsigned)));
   DMRG_HIP(hipMalloc(&cvec, kmax * elem_bytes));
-  DMRG_HIP(hipMemset(ticket, 0, 4 * sizeof(unsigned)));
   DMRG_HIP(hipMemset(alpha, 0, kmax * sizeof(double)));
   DMRG_HIP(hipMemset(beta, 0, kmax * sizeof(double)));
 }
 
 void LzDev::release() {
-  for (void* p : {(void*)alpha, (void*)beta, (void*)y, (void*)scal, (void*)status, part,
-                  (void*)ticket, cvec})
+  for (void* p : {(void*)alpha, (void*)beta, (void*)y, (void*)scal, (void*)status, part, cvec})
     if (p) (void)hipFree(p);
   alpha = beta = y = scal = nullptr;
   status = nullptr;
   part = cvec = nullptr;
-  ticket = nullptr;
 }
+
+namespace {
+// Partial-sum regions of LzDev::part (doubles): [0, nb) alpha, [nb, 2nb) norms,
+// [2nb, ...) projection coefficients, (j*NC + c)*nb + blk.
+inline double* partA(LzDev& d) { return (double*)d.part; }
+inline double* partN(LzDev& d, int nb) { return (double*)d.part + nb; }
+inline double* partC(LzDev& d, int nb) { return (double*)d.part + 2 * (size_t)nb; }
+inline void need_parts(const LzDev& d, size_t n, int cnt) {
+  const size_t need = (size_t)(cnt + 2) * lz_blocks(n) * sizeof(double);
+  DMRG_REQUIRE(need <= d.part_bytes, "lanczos: partial buffer too small");
+}
+}  // namespace
 
 template <class T>
 void lz_norm(hipStream_t st, LzDev& d, const T* x, size_t n) {
-  hipLaunchKernelGGL(k_norm<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, x, n, (double*)d.part,
-                     d.ticket, d.scal);
+  const int nb = lz_blocks(n);
+  need_parts(d, n, 0);
+  hipLaunchKernelGGL(k_norm<T>, dim3(nb), dim3(NT), 0, st, x, n, partN(d, nb));
   DMRG_LAUNCH_CHECK();
 }
 
 template <class T>
 void lz_fill_if_small(hipStream_t st, LzDev& d, T* x, size_t n, uint64_t seed, double thr) {
+  const int nb = lz_blocks(n);
   hipLaunchKernelGGL(k_fill_random<T>, dim3(std::min<size_t>(cdiv(n, NT), 2048)), dim3(NT), 0, st,
-                     x, n, seed, 0ull, 1.0, d.scal, thr);
+                     x, n, seed, 0ull, 1.0, (const double*)partN(d, nb), nb, thr);
   DMRG_LAUNCH_CHECK();
 }
 
@@ -536,38 +529,61 @@ template <class T>
 void fill_random(hipStream_t st, T* x, size_t n, uint64_t seed, uint64_t offset, double scale) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_fill_random<T>, dim3(std::min<size_t>(cdiv(n, NT), 2048)), dim3(NT), 0, st,
-                     x, n, seed, offset, scale, (const double*)nullptr, 0.0);
+                     x, n, seed, offset, scale, (const double*)nullptr, 0, 0.0);
   DMRG_LAUNCH_CHECK();
 }
 
 template <class T>
-void lz_scale(hipStream_t st, LzDev& d, T* dst, const T* src, size_t n) {
-  hipLaunchKernelGGL(k_scale<T>, dim3(std::min<size_t>(cdiv(n, NT), 2048)), dim3(NT), 0, st, dst,
-                     src, n, d.scal);
+void lz_scale(hipStream_t st, LzDev& d, T* dst, const T* src, size_t n, double* beta_out) {
+  const int nb = lz_blocks(n);
+  hipLaunchKernelGGL(k_scale<T>, dim3(nb), dim3(NT), 0, st, dst, src, n,
+                     (const double*)partN(d, nb), nb, d.scal, beta_out);
   DMRG_LAUNCH_CHECK();
+}
+
+template <class T>
+void lz_next(hipStream_t st, LzDev& d, T* vnext, const T* w, size_t n, int it) {
+  lz_scale<T>(st, d, vnext, w, n, d.beta + it);
 }
 
 template <class T>
 void lz_alpha(hipStream_t st, LzDev& d, const T* v, const T* w, size_t n, int it) {
-  hipLaunchKernelGGL(k_alpha<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, v, w, n, (double*)d.part,
-                     d.ticket, d.alpha, it);
+  (void)it;
+  const int nb = lz_blocks(n);
+  hipLaunchKernelGGL(k_alpha<T>, dim3(nb), dim3(NT), 0, st, v, w, n, partA(d));
+  DMRG_LAUNCH_CHECK();
+}
+
+void lz_alpha_value(hipStream_t st, LzDev& d, size_t n, int it) {
+  hipLaunchKernelGGL(k_alpha_value, dim3(1), dim3(NT), 0, st, (const double*)partA(d),
+                     lz_blocks(n), d.alpha, it);
   DMRG_LAUNCH_CHECK();
 }
 
 template <class T>
 void lz_sub_dots(hipStream_t st, LzDev& d, T* w, const T* V, size_t n, int it) {
   DMRG_REQUIRE(it < d.kmax && it < 128, "lanczos: Krylov dimension exceeds 128");
-  size_t need = (size_t)(it + 1) * Acc<T>::N * lz_blocks(n) * sizeof(double);
-  DMRG_REQUIRE(need <= d.part_bytes, "lanczos: partial buffer too small");
-  hipLaunchKernelGGL(k_sub_dots<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, w, V, n, it, d.alpha,
-                     d.beta, (double*)d.part, d.ticket, (T*)d.cvec);
+  need_parts(d, n, (it + 1) * Acc<T>::N);
+  const int nb = lz_blocks(n);
+  hipLaunchKernelGGL(k_sub_dots<T>, dim3(nb), dim3(NT), 0, st, w, V, n, it,
+                     (const double*)partA(d), d.alpha, (const double*)d.beta, partC(d, nb));
   DMRG_LAUNCH_CHECK();
 }
 
 template <class T>
 void lz_orth_norm(hipStream_t st, LzDev& d, T* w, const T* V, size_t n, int it) {
-  hipLaunchKernelGGL(k_orth_norm<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, w, V, n, it,
-                     (const T*)d.cvec, (double*)d.part, d.ticket, d.beta, d.scal);
+  const int nb = lz_blocks(n);
+  const int cnt = (it + 1) * Acc<T>::N;
+  // every workgroup re-reduces the coefficients while that stays small; past
+  // that one workgroup reduces them first
+  const T* cv = nullptr;
+  if ((size_t)cnt * nb > 8192) {
+    hipLaunchKernelGGL(k_coeffs, dim3(1), dim3(NT), 0, st, (const double*)partC(d, nb), nb, cnt,
+                       (double*)d.cvec);
+    cv = (const T*)d.cvec;
+  }
+  hipLaunchKernelGGL(k_orth_norm<T>, dim3(nb), dim3(NT), 0, st, w, V, n, it,
+                     (const double*)partC(d, nb), cv, partN(d, nb));
   DMRG_LAUNCH_CHECK();
 }
 
@@ -594,15 +610,17 @@ void lz_reset(hipStream_t st, LzDev& d) {
 template <class T>
 void lz_ritz(hipStream_t st, LzDev& d, T* out, const T* V, size_t n, int niter) {
   DMRG_REQUIRE(niter <= 128, "lanczos: Krylov dimension exceeds 128");
-  hipLaunchKernelGGL(k_ritz<T>, dim3(lz_blocks(n)), dim3(NT), 0, st, out, V, n, niter, d.y,
-                     (double*)d.part, d.ticket, d.scal);
+  const int nb = lz_blocks(n);
+  hipLaunchKernelGGL(k_ritz<T>, dim3(nb), dim3(NT), 0, st, out, V, n, niter, (const double*)d.y,
+                     partN(d, nb));
   DMRG_LAUNCH_CHECK();
 }
 
 #define DMRG_INST(T)                                                                        \
   template void lz_norm<T>(hipStream_t, LzDev&, const T*, size_t);                          \
   template void lz_fill_if_small<T>(hipStream_t, LzDev&, T*, size_t, uint64_t, double);     \
-  template void lz_scale<T>(hipStream_t, LzDev&, T*, const T*, size_t);                     \
+  template void lz_scale<T>(hipStream_t, LzDev&, T*, const T*, size_t, double*);            \
+  template void lz_next<T>(hipStream_t, LzDev&, T*, const T*, size_t, int);                 \
   template void lz_alpha<T>(hipStream_t, LzDev&, const T*, const T*, size_t, int);          \
   template void lz_sub_dots<T>(hipStream_t, LzDev&, T*, const T*, size_t, int);             \
   template void lz_orth_norm<T>(hipStream_t, LzDev&, T*, const T*, size_t, int);            \
