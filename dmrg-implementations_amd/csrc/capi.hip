@@ -55,6 +55,7 @@ Options to_opts(const dmrg_options* o) {
     r.svd_tol = o->svd_tol;
     r.svd_warm = o->svd_warm;
     r.fp32_eig_rel = o->fp32_eig_rel;
+    r.svd_subspace = o->svd_subspace;
   }
   return r;
 }
@@ -89,6 +90,7 @@ void dmrg_default_options(dmrg_options* o) {
   o->svd_tol = d.svd_tol;
   o->svd_warm = d.svd_warm;
   o->fp32_eig_rel = d.fp32_eig_rel;
+  o->svd_subspace = d.svd_subspace;
 }
 
 const char* dmrg_last_error(void) { return g_err.c_str(); }

@@ -29,6 +29,9 @@ struct Options {
   // FP32 matvec mode: the Lanczos Ritz value cannot settle below the FP32
   // rounding of H_eff, so the check also accepts |dE| < fp32_eig_rel |E|.
   double fp32_eig_rel = 1e-9;
+  // Truncation: converge only the kept/discarded coupling of each split
+  // (same truncated state; the kept vectors are a basis of the kept space).
+  int svd_subspace = 1;
 };
 
 struct Stats {
@@ -146,6 +149,7 @@ class Engine {
     SvdParams p;
     p.max_sweeps = opt.svd_max_sweeps;
     p.tol = opt.svd_tol;
+    p.subspace = opt.svd_subspace != 0;
     return p;
   }
   LanczosRunner<T> runner;

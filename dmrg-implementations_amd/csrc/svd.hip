@@ -475,8 +475,10 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 // max |A(i,j)|, i != j, over the N x N block -> atomicMax on the bit pattern
 // (non-negative doubles order like their bits; NaN maps to +inf bits + 1).
+// With rank given, only pairs with exactly one of rank[i], rank[j] < keep.
 template <class T>
 __global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, int Np,
+                                                const int* __restrict__ rank, int keep,
                                                 unsigned long long* out) {
   __shared__ double red[256];
   const int t = threadIdx.x;
@@ -485,6 +487,7 @@ __global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, 
   for (size_t e = blockIdx.x * (size_t)blockDim.x + t; e < tot; e += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(e % Np), c = (int)(e / Np);
     if (r == c || r >= N || c >= N) continue;
+    if (rank && ((rank[r] < keep) == (rank[c] < keep))) continue;
     double v = absv(A[e]);
     if (!(v == v)) v = INFINITY;
     mx = fmax(mx, v);
@@ -501,6 +504,12 @@ __global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, 
     memcpy(&bits, &v, sizeof(v));
     atomicMax(out, bits);
   }
+}
+
+// rank[perm[p]] = p, p < n.
+__global__ void k_invperm(const int* __restrict__ perm, int n, int* __restrict__ rank) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+    rank[perm[p]] = p;
 }
 
 // out(i, j) = V(i, perm[j]) (conj-transposed into out(j, i) with `herm`).
@@ -534,7 +543,8 @@ __global__ void k_gather(const T* __restrict__ V, int ldv, const int* __restrict
 template <class T>
 __global__ __launch_bounds__(256) void k_qr_sigma(const T* __restrict__ G, int ldg, int pass,
                                                   const double* __restrict__ r0,
-                                                  double* __restrict__ out) {
+                                                  double* __restrict__ out,
+                                                  double* __restrict__ gdiag) {
   __shared__ double red[256];
   const int j = blockIdx.x, t = threadIdx.x;
   double s = 0.0;
@@ -549,8 +559,10 @@ __global__ __launch_bounds__(256) void k_qr_sigma(const T* __restrict__ G, int l
     __syncthreads();
   }
   if (t == 0) {
-    const double r = real(G[j + (size_t)j * ldg]) - red[0];
+    const double gjj = real(G[j + (size_t)j * ldg]);
+    const double r = gjj - red[0];
     out[j] = pass == 0 ? r : sqrt(fmax(r, 0.0));
+    if (gdiag) gdiag[j] = gjj;
   }
 }
 
@@ -607,9 +619,11 @@ __global__ __launch_bounds__(256) void k_vecnorm(const T* __restrict__ Y, int ld
 }
 
 // Truncation rank (scalar_traits.h:497-505 semantics, plus the optional
-// relative cutoff) and discarded weight trace - sum_{j<nk} sigma_j^2.
-__global__ void k_rank(const double* __restrict__ sigma, int k, double cutoff, double rel,
-                       double* scal, int* newk) {
+// relative cutoff) and discarded weight trace - sum_{j<nk} |product vector j|^2
+// (the weight the kept orthonormal vectors carry, exact for any orthonormal
+// basis of the kept space).
+__global__ void k_rank(const double* __restrict__ sigma, const double* __restrict__ gdiag, int k,
+                       double cutoff, double rel, double* scal, int* newk) {
   const double thr = fmax(cutoff, rel * sigma[0]);
   int nk = k;
   for (int i = 0; i < k; ++i) {
@@ -618,7 +632,7 @@ __global__ void k_rank(const double* __restrict__ sigma, int k, double cutoff, d
   }
   if (nk == 0) nk = 1;
   double kept = 0.0;
-  for (int i = 0; i < nk; ++i) kept += sigma[i] * sigma[i];
+  for (int i = 0; i < nk; ++i) kept += gdiag[i];
   scal[1] = fmax(scal[2] - kept, 0.0);
   *newk = nk;
 }
@@ -692,9 +706,16 @@ inline unsigned grid_for(size_t tot) { return (unsigned)std::min<size_t>(cdiv(to
 // Block-Jacobi eigen-decomposition of the Hermitian matrix already stored in
 // w.A[0] (N x N, ld Np, padding arbitrary).  Leaves the diagonalised matrix in
 // w.A[cur], the eigenvectors in w.V[cur] and the descending order in w.perm.
+// keep > 0: converged once the `keep` largest diagonal entries decouple from
+// the rest (max |A_ij| over those pairs <= tol |A|_F); couplings inside either
+// group are left as they are.  When the keep-th largest diagonal entry comes
+// within a factor 1e4 of max(cut_abs^2, cut_rel^2 * largest) -- the cutoff
+// may then drop kept vectors, which needs them resolved -- the iteration goes
+// on to full convergence.
 template <class T>
 int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, int& sweeps,
-           double& off_rel, bool v_given = false) {
+           double& off_rel, bool v_given = false, int keep = 0, double cut_abs = 0.0,
+           double cut_rel = 0.0) {
   const int nb = Np / JB;
   const int np = nb / 2;
   const size_t tot = (size_t)Np * Np;
@@ -719,31 +740,56 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
     DMRG_LAUNCH_CHECK();
     return cur;
   }
+  double key_hi = 0.0, key_keep = 0.0;  // largest and keep-th largest diagonal entry
   auto offmax = [&](const T* M) {
+    if (keep > 0) {
+      hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, M, Np, nullptr, Np, N, w.perm,
+                         w.resid);
+      hipLaunchKernelGGL(k_invperm, dim3(cdiv(Np, 256)), dim3(256), 0, st, (const int*)w.perm, Np,
+                         w.rank);
+    }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
     hipLaunchKernelGGL(k_offmax<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0, st,
-                       M, N, Np, w.off);
+                       M, N, Np, keep > 0 ? (const int*)w.rank : nullptr, keep, w.off);
     DMRG_LAUNCH_CHECK();
     unsigned long long bits = 0;
     DMRG_HIP(hipMemcpyAsync(&bits, w.off, sizeof(bits), hipMemcpyDeviceToHost, st));
+    if (keep > 0) {
+      DMRG_HIP(hipMemcpyAsync(&key_hi, w.resid, sizeof(double), hipMemcpyDeviceToHost, st));
+      DMRG_HIP(hipMemcpyAsync(&key_keep, w.resid + (keep - 1), sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+    }
     DMRG_HIP(hipStreamSynchronize(st));
     double o;
     std::memcpy(&o, &bits, sizeof(o));
     if (std::isinf(o) || !(o == o)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
     return o;
   };
+  // subspace convergence reached: accept it, or switch to full convergence
+  // when the cutoff reaches into the kept group (returns the full off-norm).
+  auto accept_keep = [&](const T* M, double o) {
+    if (keep == 0) return o;
+    const double floor = 1e4 * std::max(cut_abs * cut_abs, cut_rel * cut_rel * key_hi);
+    if (key_keep >= floor) return o;
+    keep = 0;
+    return offmax(M);
+  };
   bool done = false;
   if (v_given) {
     // Started from a basis near the eigenbasis: up to two first-order
-    // (all-pairs, GEMM-only) corrections before any Jacobi sweep.
+    // (all-pairs, GEMM-only) corrections before any Jacobi sweep.  A pair is
+    // corrected when its gap exceeds sep |A_ij| (the first-order step then
+    // leaves O(|A_ij|/sep)); the subspace criterion tolerates the rougher sep.
     const GemmTile tl = gemm_pick_tile(Np, Np, Np, is_cplx<T>::value);
+    const double sep = keep > 0 ? 1e2 : 1e4;
     for (int it = 0; it < 2 && !done; ++it) {
-      const double o = offmax(w.A[0]);
+      double o = offmax(w.A[0]);
+      if (o <= prm.tol * fro) o = accept_keep(w.A[0], o);
       off_rel = o / fro;
       if (o <= prm.tol * fro) { done = true; break; }
       if (o > 1e-6 * fro) break;  // not perturbative: Jacobi
       hipLaunchKernelGGL(k_oa_q<T>, dim3(grid_for(tot)), dim3(256), 0, st, (const T*)w.A[0], w.Y,
-                         N, Np, 1e4);
+                         N, Np, sep);
       // make Q unitary: Newton-Schulz Q <- Q (3I - Q^H Q)/2 until max|Q^H Q - I|
       // was below 1e-8 before the last step (the error then squares to < 1e-16)
       bool unitary = false;
@@ -785,18 +831,10 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
       hipLaunchKernelGGL(k_pair_apply<T>, dim3(noff + nvt), dim3(NT), 0, st, w.A[cur], w.V[cur], Np,
                          nb, r, (const T*)w.Q, (const int*)w.qflag);
     }
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
-    hipLaunchKernelGGL(k_offmax<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0, st,
-                       w.A[cur], N, Np, w.off);
-    DMRG_LAUNCH_CHECK();
-    unsigned long long bits = 0;
-    DMRG_HIP(hipMemcpyAsync(&bits, w.off, sizeof(bits), hipMemcpyDeviceToHost, st));
-    DMRG_HIP(hipStreamSynchronize(st));
-    double off;
-    std::memcpy(&off, &bits, sizeof(off));
+    double off = offmax(w.A[cur]);
+    if (off <= prm.tol * fro) off = accept_keep(w.A[cur], off);
     sweeps = sw + 1;
     off_rel = off / fro;
-    if (std::isinf(off) || !(off == off)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
     if (off <= prm.tol * fro) { done = true; break; }
   }
   if (!done) throw Error(ST_SOLVER_NO_CONV, "svd: Jacobi eigen-solver did not converge");
@@ -822,6 +860,8 @@ void SvdWork<T>::alloc(int cap_n) {
   DMRG_HIP(hipMalloc(&Q, (size_t)(cap / JP) * JP * JP * sizeof(T)));
   DMRG_HIP(hipMalloc(&qflag, (cap / JP) * sizeof(int)));
   DMRG_HIP(hipMalloc(&perm, cap * sizeof(int)));
+  DMRG_HIP(hipMalloc(&rank, cap * sizeof(int)));
+  DMRG_HIP(hipMalloc(&gdiag, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&sigma, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&resid, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&scal, 4 * sizeof(double)));
@@ -836,13 +876,15 @@ void SvdWork<T>::release() {
     if (V[i]) (void)hipFree(V[i]);
     A[i] = V[i] = nullptr;
   }
-  void* ps[] = {Y, Q, qflag, perm, sigma, resid, scal, off, newk};
+  void* ps[] = {Y, Q, qflag, perm, rank, gdiag, sigma, resid, scal, off, newk};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   Y = nullptr;
   Q = nullptr;
   qflag = nullptr;
   perm = nullptr;
+  rank = nullptr;
+  gdiag = nullptr;
   sigma = nullptr;
   resid = nullptr;
   scal = nullptr;
@@ -878,7 +920,9 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
   }
   SvdResult res;
   res.N = N;
-  const int cur = jacobi<T>(st, w, N, Np, prm, res.sweeps, res.off, warm != nullptr);
+  const int keep = (prm.subspace && k < N) ? k : 0;
+  const int cur = jacobi<T>(st, w, N, Np, prm, res.sweeps, res.off, warm != nullptr, keep, cutoff,
+                            rel_cutoff);
   if (basis_out)
     hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)N * N)), dim3(256), 0, st,
                        (const T*)w.V[cur], Np, (const int*)w.perm, N, N, 0, basis_out, N);
@@ -903,7 +947,7 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
   // clean pivots, computed on the vectors themselves (absolute accuracy
   // ~eps |theta|, like |R_jj| of a QR), pivots chosen from the Gram matrix.
   hipLaunchKernelGGL(k_qr_sigma<T>, dim3(k), dim3(256), 0, st, (const T*)w.A[1 - cur], k, 0,
-                     (const double*)nullptr, w.resid);
+                     (const double*)nullptr, w.resid, w.gdiag);
   {
     const bool byr = !scale_left;            // 'U' side: vectors are the rows of Y (k x n)
     const T* Yv = byr ? w.Y : left;
@@ -936,14 +980,24 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
       hipLaunchKernelGGL(k_vecnorm<T>, dim3(k), dim3(256), 0, st, (const T*)R, m, len, 0, w.sigma);
     }
   }
-  hipLaunchKernelGGL(k_rank, dim3(1), dim3(1), 0, st, (const double*)w.sigma, k, cutoff, rel_cutoff,
-                     w.scal, w.newk);
+  hipLaunchKernelGGL(k_rank, dim3(1), dim3(1), 0, st, (const double*)w.sigma,
+                     (const double*)w.gdiag, k, cutoff, rel_cutoff, w.scal, w.newk);
   DMRG_LAUNCH_CHECK();
   int nk = 0;
   double disc = 0.0;
   DMRG_HIP(hipMemcpyAsync(&nk, w.newk, sizeof(int), hipMemcpyDeviceToHost, st));
   DMRG_HIP(hipMemcpyAsync(&disc, w.scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
   DMRG_HIP(hipStreamSynchronize(st));
+  if (keep > 0 && nk < k) {
+    // the cutoff cuts inside the kept group, whose vectors are not resolved:
+    // redo the split with every pair converged
+    SvdParams full = prm;
+    full.subspace = false;
+    SvdResult r2 = svd_split<T>(st, w, A, m, n, lda, kmax, cutoff, rel_cutoff, scale_left, left,
+                                ldl, right, sigma_out, full, warm, basis_out);
+    r2.sweeps += res.sweeps;
+    return r2;
+  }
   res.nk = nk;
   res.discarded = disc;
   if (!scale_left)

@@ -44,6 +44,8 @@ struct SvdWork {
   T* Q = nullptr;                // per-pair 32x32 rotations of one round
   int* qflag = nullptr;          // per pair: 1 if its Q is not the identity
   int* perm = nullptr;           // sort permutation
+  int* rank = nullptr;           // inverse of perm (position of each index)
+  double* gdiag = nullptr;       // squared norms of the k product vectors
   double* sigma = nullptr;       // singular values in output order
   double* resid = nullptr;       // first-pass QR residuals
   double* scal = nullptr;        // [0] |rho|_F, [1] discarded weight, [2] trace(rho)
@@ -57,6 +59,14 @@ struct SvdWork {
 struct SvdParams {
   int max_sweeps = 40;  // outer block-Jacobi sweeps
   double tol = 1e-14;   // stop when max |rho_ij| <= tol * |rho|_F
+  // svd_split only: converge just the coupling between the k leading
+  // eigenvectors (k = min(m, n, kmax)) and the rest, i.e. the kept subspace,
+  // not the individual vectors inside it.  The truncated product left*right
+  // is the same (to tol); left is an orthonormal basis of the kept subspace
+  // rather than the singular vectors, and sigma_out holds the norms of the
+  // product vectors after projecting out the earlier ones.  Falls back to
+  // full convergence when the cutoff keeps fewer than k vectors.
+  bool subspace = false;
 };
 
 struct SvdResult {

@@ -51,7 +51,8 @@ class Options(C.Structure):
                 ("svd_cutoff", C.c_double), ("svd_rel_cutoff", C.c_double),
                 ("precision", C.c_int), ("profile", C.c_int), ("device", C.c_int),
                 ("svd_max_sweeps", C.c_int), ("svd_tol", C.c_double),
-                ("svd_warm", C.c_int), ("fp32_eig_rel", C.c_double)]
+                ("svd_warm", C.c_int), ("fp32_eig_rel", C.c_double),
+                ("svd_subspace", C.c_int)]
 
 
 class Stats(C.Structure):

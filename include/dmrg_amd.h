@@ -64,6 +64,11 @@ typedef struct dmrg_options {
   double fp32_eig_rel;   /* FP32 matvec mode only: Lanczos also converges when
                             |dE| < fp32_eig_rel*|E| (FP32 rounding of H_eff
                             bounds how far the Ritz value settles); 1e-9 */
+  int svd_subspace;      /* 1 (default): a split's Jacobi stops once the kept
+                            chi vectors decouple from the discarded ones; the
+                            truncated state is the same, the kept tensor is an
+                            orthonormal basis of the kept space rather than the
+                            singular vectors.  0: converge every pair.       */
 } dmrg_options;
 
 typedef struct dmrg_stats {

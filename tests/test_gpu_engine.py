@@ -180,3 +180,35 @@ def test_svd_warm_start_matches_cold(dmrg_amd, cplx):
     assert np.max(np.abs(out[1][0] - out[0][0])) < 1e-10 * L
     # converged sweeps need fewer Jacobi sweeps per split when warm-started
     assert out[1][1]["svd_sweeps"] < out[0][1]["svd_sweeps"]
+
+
+@pytest.mark.parametrize("cplx,init", [(False, "random"), (True, "random"), (False, "neel")])
+def test_svd_subspace_matches_full(dmrg_amd, cplx, init):
+    """Stopping each split's Jacobi once the kept chi vectors decouple from the
+    discarded ones gives the same truncated state as converging every pair:
+    sweep energies agree and the MPS stays normalised.  The Neel start has
+    rank-deficient two-site tensors, where the cutoff keeps fewer than chi
+    vectors and the split falls back to full convergence."""
+    L, chi = 12, 24
+    if cplx:
+        Ws = models.josephson_mpo(L, n_max=1, E_J=1.0, E_C=0.5, phi_ext=np.pi / 4)
+        d, D, dt = 3, 4, "c128"
+    else:
+        Ws = models.heisenberg_mpo(L)
+        d, D, dt = 2, 5, "f64"
+    out = {}
+    for sub in (0, 1):
+        eng = dmrg_amd.DMRG(L, d, chi, D, dtype=dt, tol=0.0, krylov_max=30, svd_subspace=sub)
+        eng.set_mpo(Ws)
+        if init == "neel":
+            eng.initialize_mps_neel()
+        else:
+            eng.initialize_mps_random(seed=37, scale=0.1)
+        eng.run(5)
+        out[sub] = (np.array(eng.sweep_energies()), eng.stats())
+        e, n2 = eng.expectation()
+        assert abs(n2 - 1) < 1e-10
+        eng.close()
+    assert np.max(np.abs(out[1][0] - out[0][0])) < 1e-10 * L
+    if not cplx and init == "random":
+        assert out[1][1]["svd_sweeps"] <= out[0][1]["svd_sweeps"]
