@@ -96,7 +96,8 @@ def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
                heff_tflops=heff_tf, heff_ms=st["heff_ms"], heff_flop=st["heff_flop"],
                matvecs=st["matvecs"], svd_ms=st["svd_ms"], lanczos_ms=st["lanczos_ms"],
                svd_calls=st["svd_calls"], svd_sweeps=st["svd_sweeps"],
-               svd_warm_starts=st["svd_warm_starts"], sweep_s_each=per_sweep,
+               svd_warm_starts=st["svd_warm_starts"], svd_bip_sweeps=st["svd_bip_sweeps"],
+               svd_oa_steps=st["svd_oa_steps"], sweep_s_each=per_sweep,
                sweep_s_median=sorted(per_sweep)[len(per_sweep) // 2] if per_sweep else 0.0,
                ms_per_matvec=st["heff_ms"] / max(st["matvecs"], 1))
     return eng, res

@@ -265,6 +265,8 @@ int dmrg_get_stats(dmrg_handle h, dmrg_stats* st) {
       st->env_build_s = s.env_build_s;
       st->sweeps_s_total = s.sweeps_s_total;
       st->svd_warm_starts = s.svd_warm_starts;
+      st->svd_bip_sweeps = s.svd_bip_sweeps;
+      st->svd_oa_steps = s.svd_oa_steps;
     });
   });
 }

@@ -427,6 +427,8 @@ void Engine<T>::split_two(int site, char dir) {
   stats.svd_ms += 1e3 * (now_s() - t0);
   stats.svd_calls += 1;
   stats.svd_sweeps += r.sweeps;
+  stats.svd_bip_sweeps += r.bip_sweeps;
+  stats.svd_oa_steps += r.oa_steps;
   stats.svd_warm_starts += warm ? 1 : 0;
   stats.max_discarded = std::max(stats.max_discarded, r.discarded);
 }
@@ -451,7 +453,7 @@ void Engine<T>::warm_alloc() {
   const size_t tsz = (size_t)chi * chi;
   for (T** p : {&wT[0], &wT[1], &wZ0, &wZ1, &wP, &wTmp}) dmalloc((void**)p, tsz * sizeof(T));
   dmalloc((void**)&wXw, capN * capN * sizeof(T));
-  dmalloc((void**)&wDev, sizeof(double));
+  dmalloc((void**)&wDev, kPolarMaxIter * sizeof(double));
   warm_on = true;
 }
 

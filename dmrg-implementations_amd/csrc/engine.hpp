@@ -42,6 +42,7 @@ struct Stats {
   double max_discarded = 0;
   double env_build_s = 0, sweeps_s_total = 0;
   long long svd_warm_starts = 0;
+  long long svd_bip_sweeps = 0, svd_oa_steps = 0;
 };
 
 // Algorithmic flop counts (real arithmetic; complex = 4x).

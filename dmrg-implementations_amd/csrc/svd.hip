@@ -24,6 +24,38 @@ __device__ __host__ inline void rr_pair(int nb, int r, int k, int& a, int& b) {
   if (k == 0) { a = r; b = n1; }
   else { a = (r + k) % n1; b = (r - k + n1) % n1; }
 }
+// Block pair k of round r under schedule ka.  ka == 0: round robin over all
+// nb blocks (nb-1 rounds cover every pair).  ka > 0: the first ka blocks
+// (the kept group after the sort) against the other nb-ka ("bipartite"; only
+// couplings across the two groups are rotated): with a = min(ka, nb-ka) on the
+// smaller side S and b = max on the larger side G, pair k < a is
+// (S_k, G_{(k+r) mod b}), so b rounds meet every S-G pair once; pairs k >= a
+// join the G blocks left over in round r two by two and are not rotated
+// (pair_is_idle).  Every round is a perfect matching of the nb blocks.
+__device__ __host__ inline void sched_pair(int nb, int ka, int r, int k, int& I, int& J) {
+  if (ka == 0) { rr_pair(nb, r, k, I, J); return; }
+  const int kb = nb - ka;
+  const int a = ka < kb ? ka : kb, b = nb - a;
+  const int s0 = ka < kb ? 0 : ka, g0 = ka < kb ? ka : 0;  // first block of S and of G
+  if (k < a) {
+    I = s0 + k;
+    J = g0 + (k + r) % b;
+  } else {
+    const int q = k - a;
+    I = g0 + (a + 2 * q + r) % b;
+    J = g0 + (a + 2 * q + 1 + r) % b;
+  }
+}
+__device__ __host__ inline bool pair_is_idle(int nb, int ka, int k) {
+  if (ka == 0) return false;
+  const int kb = nb - ka;
+  return k >= (ka < kb ? ka : kb);
+}
+// Rounds of one sweep under schedule ka.
+inline int sched_rounds(int nb, int ka) {
+  if (ka == 0) return nb - 1;
+  return std::max(ka, nb - ka);
+}
 // Global row/column of local index i (0..JP-1) of the pair (I, J).
 __device__ inline int gidx(int I, int J, int i) { return i < JB ? I * JB + i : J * JB + (i - JB); }
 // Inner circle schedule over the JP local indices: round rd pairs positions
@@ -74,23 +106,69 @@ __global__ void k_set_basis(T* __restrict__ V, const T* __restrict__ X, int N, i
   }
 }
 
-// scal[0] = |rho|_F, scal[2] = trace(rho) (one workgroup).
+// Partial sums of |rho|_F^2 and trace(rho): red[blk] and red[NRED + blk].
+constexpr int NRED = 256;  // max workgroups of k_norms_part
 template <class T>
-__global__ __launch_bounds__(1024) void k_norms(const T* __restrict__ A, int N, int Np,
-                                                double* scal) {
-  __shared__ double s1[1024], s2[1024];
+__global__ __launch_bounds__(256) void k_norms_part(const T* __restrict__ A, int N, int Np,
+                                                    double* __restrict__ red) {
+  __shared__ double s1[256], s2[256];
   const int t = threadIdx.x;
   double f = 0.0, tr = 0.0;
-  for (size_t e = t; e < (size_t)Np * Np; e += 1024) f += abs2(A[e]);
-  for (int i = t; i < N; i += 1024) tr += real(A[i + (size_t)i * Np]);
+  const size_t tot = (size_t)Np * Np;
+  for (size_t e = blockIdx.x * (size_t)256 + t; e < tot; e += (size_t)gridDim.x * 256)
+    f += abs2(A[e]);
+  for (int i = blockIdx.x * 256 + t; i < N; i += gridDim.x * 256) tr += real(A[i + (size_t)i * Np]);
   s1[t] = f;
   s2[t] = tr;
   __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) { s1[t] += s1[t + o]; s2[t] += s2[t + o]; }
+    __syncthreads();
+  }
+  if (t == 0) { red[blockIdx.x] = s1[0]; red[NRED + blockIdx.x] = s2[0]; }
+}
+// scal[0] = |rho|_F, scal[2] = trace(rho) from nb partials (one workgroup,
+// fixed order).
+__global__ __launch_bounds__(256) void k_norms_fin(const double* __restrict__ red, int nb,
+                                                   double* scal) {
+  __shared__ double s1[256], s2[256];
+  const int t = threadIdx.x;
+  s1[t] = t < nb ? red[t] : 0.0;
+  s2[t] = t < nb ? red[NRED + t] : 0.0;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
     if (t < o) { s1[t] += s1[t + o]; s2[t] += s2[t + o]; }
     __syncthreads();
   }
   if (t == 0) { scal[0] = sqrt(s1[0]); scal[2] = s2[0]; }
+}
+
+// rank[i] = position of diagonal entry i in the stable descending order of
+// k_sort (ties by index, NaN as -inf); padding i >= N gets rank i.
+// keys[0] = largest entry, keys[1] = entry of rank keep-1.
+template <class T>
+__global__ __launch_bounds__(256) void k_rank_diag(const T* __restrict__ A, int N, int Np,
+                                                   int keep, int* __restrict__ rank,
+                                                   double* __restrict__ keys) {
+  __shared__ double dg[SORT_MAX];
+  const int t = threadIdx.x;
+  for (int i = t; i < N; i += 256) {
+    double v = real(A[i + (size_t)i * Np]);
+    dg[i] = (v == v) ? v : -INFINITY;
+  }
+  __syncthreads();
+  for (int i = blockIdx.x * 256 + t; i < Np; i += gridDim.x * 256) {
+    if (i >= N) { rank[i] = i; continue; }
+    const double di = dg[i];
+    int r = 0;
+    for (int j = 0; j < N; ++j) {
+      const double dj = dg[j];
+      r += (dj > di) || (dj == di && j < i);
+    }
+    rank[i] = r;
+    if (r == 0) keys[0] = di;
+    if (r == keep - 1) keys[1] = di;
+  }
 }
 
 // Stable descending sort of n <= 2048 keys -> perm (and the sorted keys).
@@ -171,8 +249,9 @@ __global__ void k_permute(const T* __restrict__ A, const T* __restrict__ V, T* _
 constexpr int NTA = 256;
 
 template <class T>
-__global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int nb, int round,
-                                                  T* __restrict__ Qout, int* __restrict__ qflag,
+__global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int nb, int ka,
+                                                  int round, T* __restrict__ Qout,
+                                                  int* __restrict__ qflag,
                                                   const double* __restrict__ scal, double rot_tol,
                                                   int inner_max) {
   __shared__ T As[JP * LDP];  // As[c * LDP + r] = sub(r, c)
@@ -182,8 +261,12 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
   __shared__ int rf[JP / 2];
   __shared__ int s_rot;
   const int k = blockIdx.x, t = threadIdx.x;
+  if (pair_is_idle(nb, ka, k)) {
+    if (t == 0) qflag[k] = 0;
+    return;
+  }
   int I, J;
-  rr_pair(nb, round, k, I, J);
+  sched_pair(nb, ka, round, k, I, J);
   const double fro = scal[0];
   const double thr = rot_tol * fro;
   const double thr2 = thr * thr;
@@ -397,7 +480,8 @@ __device__ inline void acc_to_lds(double* __restrict__ Xs, d4_t (&acc)[NP<T>::P]
 
 template <class T>
 __global__ __launch_bounds__(NT) void k_pair_apply(T* __restrict__ A, T* __restrict__ V, int ld,
-                                                   int nb, int round, const T* __restrict__ Q,
+                                                   int nb, int ka, int round,
+                                                   const T* __restrict__ Q,
                                                    const int* __restrict__ qflag) {
   constexpr int P = NP<T>::P;
   constexpr int PS = JP * LDP;
@@ -415,8 +499,8 @@ __global__ __launch_bounds__(NT) void k_pair_apply(T* __restrict__ A, T* __restr
     const int f1 = qflag[p1], f2 = qflag[p2];
     if (!(f1 | f2)) return;
     int I1, J1, I2, J2;
-    rr_pair(nb, round, p1, I1, J1);
-    rr_pair(nb, round, p2, I2, J2);
+    sched_pair(nb, ka, round, p1, I1, J1);
+    sched_pair(nb, ka, round, p2, I2, J2);
     for (int e = t; e < JP * JP; e += NT) {
       const int r = e % JP, c = e / JP;
       put(Xs, c * LDP + r, PS, A[(size_t)gidx(I1, J1, r) + (size_t)gidx(I2, J2, c) * ld]);
@@ -451,7 +535,7 @@ __global__ __launch_bounds__(NT) void k_pair_apply(T* __restrict__ A, T* __restr
     const int pp = bid % np, rt = bid / np;
     if (!qflag[pp]) return;
     int I, J;
-    rr_pair(nb, round, pp, I, J);
+    sched_pair(nb, ka, round, pp, I, J);
     const size_t r0 = (size_t)rt * JP;
     for (int e = t; e < JP * JP; e += NT) {
       const int r = e % JP, c = e / JP;
@@ -504,12 +588,6 @@ __global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, 
     memcpy(&bits, &v, sizeof(v));
     atomicMax(out, bits);
   }
-}
-
-// rank[perm[p]] = p, p < n.
-__global__ void k_invperm(const int* __restrict__ perm, int n, int* __restrict__ rank) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
-    rank[perm[p]] = p;
 }
 
 // out(i, j) = V(i, perm[j]) (conj-transposed into out(j, i) with `herm`).
@@ -622,19 +700,37 @@ __global__ __launch_bounds__(256) void k_vecnorm(const T* __restrict__ Y, int ld
 // relative cutoff) and discarded weight trace - sum_{j<nk} |product vector j|^2
 // (the weight the kept orthonormal vectors carry, exact for any orthonormal
 // basis of the kept space).
-__global__ void k_rank(const double* __restrict__ sigma, const double* __restrict__ gdiag, int k,
-                       double cutoff, double rel, double* scal, int* newk) {
+__global__ __launch_bounds__(256) void k_rank(const double* __restrict__ sigma,
+                                              const double* __restrict__ gdiag, int k,
+                                              double cutoff, double rel, double* scal, int* newk) {
+  __shared__ int first[256];
+  __shared__ double part[256];
+  const int t = threadIdx.x;
   const double thr = fmax(cutoff, rel * sigma[0]);
-  int nk = k;
-  for (int i = 0; i < k; ++i) {
+  int f = k;  // first index failing the cutoff
+  for (int i = t; i < k; i += 256) {
     const double s = sigma[i];
-    if (s < thr || s != s) { nk = i; break; }
+    if (s < thr || s != s) { f = i; break; }
   }
-  if (nk == 0) nk = 1;
+  first[t] = f;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) first[t] = min(first[t], first[t + o]);
+    __syncthreads();
+  }
+  const int nk = first[0] == 0 ? 1 : first[0];
   double kept = 0.0;
-  for (int i = 0; i < nk; ++i) kept += gdiag[i];
-  scal[1] = fmax(scal[2] - kept, 0.0);
-  *newk = nk;
+  for (int i = t; i < nk; i += 256) kept += gdiag[i];
+  part[t] = kept;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) part[t] += part[t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    scal[1] = fmax(scal[2] - part[0], 0.0);
+    *newk = nk;
+  }
 }
 
 // out(j, c) = Y(j, c) for j < nk (ld nk) from Y (ld ldy).
@@ -712,16 +808,25 @@ inline unsigned grid_for(size_t tot) { return (unsigned)std::min<size_t>(cdiv(to
 // within a factor 1e4 of max(cut_abs^2, cut_rel^2 * largest) -- the cutoff
 // may then drop kept vectors, which needs them resolved -- the iteration goes
 // on to full convergence.
+struct JacobiCount {
+  int sweeps = 0, bip = 0, oa = 0;
+};
+
 template <class T>
-int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, int& sweeps,
+int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, JacobiCount& cnt,
            double& off_rel, bool v_given = false, int keep = 0, double cut_abs = 0.0,
            double cut_rel = 0.0) {
+  int& sweeps = cnt.sweeps;
   const int nb = Np / JB;
   const int np = nb / 2;
   const size_t tot = (size_t)Np * Np;
   hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[0], w.V[0], N, Np,
                      v_given ? 0 : 1);
-  hipLaunchKernelGGL(k_norms<T>, dim3(1), dim3(1024), 0, st, w.A[0], N, Np, w.scal);
+  {
+    const int nbr = (int)std::min<size_t>(NRED, std::max<size_t>(1, cdiv(tot, 4096)));
+    hipLaunchKernelGGL(k_norms_part<T>, dim3(nbr), dim3(256), 0, st, w.A[0], N, Np, w.red);
+    hipLaunchKernelGGL(k_norms_fin, dim3(1), dim3(256), 0, st, (const double*)w.red, nbr, w.scal);
+  }
   DMRG_LAUNCH_CHECK();
   double hs[3] = {0, 0, 0};
   DMRG_HIP(hipMemcpyAsync(hs, w.scal, sizeof(hs), hipMemcpyDeviceToHost, st));
@@ -742,24 +847,23 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
   }
   double key_hi = 0.0, key_keep = 0.0;  // largest and keep-th largest diagonal entry
   auto offmax = [&](const T* M) {
-    if (keep > 0) {
-      hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, M, Np, nullptr, Np, N, w.perm,
-                         w.resid);
-      hipLaunchKernelGGL(k_invperm, dim3(cdiv(Np, 256)), dim3(256), 0, st, (const int*)w.perm, Np,
-                         w.rank);
-    }
+    if (keep > 0)
+      hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, M, N, Np, keep,
+                         w.rank, w.red);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
     hipLaunchKernelGGL(k_offmax<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0, st,
                        M, N, Np, keep > 0 ? (const int*)w.rank : nullptr, keep, w.off);
     DMRG_LAUNCH_CHECK();
     unsigned long long bits = 0;
+    double kk[2] = {0.0, 0.0};
     DMRG_HIP(hipMemcpyAsync(&bits, w.off, sizeof(bits), hipMemcpyDeviceToHost, st));
-    if (keep > 0) {
-      DMRG_HIP(hipMemcpyAsync(&key_hi, w.resid, sizeof(double), hipMemcpyDeviceToHost, st));
-      DMRG_HIP(hipMemcpyAsync(&key_keep, w.resid + (keep - 1), sizeof(double),
-                              hipMemcpyDeviceToHost, st));
-    }
+    if (keep > 0)
+      DMRG_HIP(hipMemcpyAsync(kk, w.red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     DMRG_HIP(hipStreamSynchronize(st));
+    if (keep > 0) {
+      key_hi = kk[0];
+      key_keep = kk[1];
+    }
     double o;
     std::memcpy(&o, &bits, sizeof(o));
     if (std::isinf(o) || !(o == o)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
@@ -775,37 +879,48 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
     return offmax(M);
   };
   bool done = false;
+  double last_off = INFINITY;  // latest convergence measure (criterion in force)
   if (v_given) {
     // Started from a basis near the eigenbasis: up to two first-order
     // (all-pairs, GEMM-only) corrections before any Jacobi sweep.  A pair is
     // corrected when its gap exceeds sep |A_ij| (the first-order step then
     // leaves O(|A_ij|/sep)); the subspace criterion tolerates the rougher sep.
+    // Steps go on while each one cuts the measure at least tenfold.
     const GemmTile tl = gemm_pick_tile(Np, Np, Np, is_cplx<T>::value);
     const double sep = keep > 0 ? 1e2 : 1e4;
-    for (int it = 0; it < 2 && !done; ++it) {
+    const double gate = keep > 0 ? 1e-4 : 1e-6;
+    const int max_steps = keep > 0 ? 3 : 2;
+    for (int it = 0; it <= max_steps && !done; ++it) {
       double o = offmax(w.A[0]);
       if (o <= prm.tol * fro) o = accept_keep(w.A[0], o);
+      const double prev = last_off;
+      last_off = o;
       off_rel = o / fro;
       if (o <= prm.tol * fro) { done = true; break; }
-      if (o > 1e-6 * fro) break;  // not perturbative: Jacobi
+      if (it == max_steps || o > gate * fro) break;  // not perturbative: Jacobi
+      if (it > 0 && o > 0.1 * prev) break;           // stalled (clusters): Jacobi
       hipLaunchKernelGGL(k_oa_q<T>, dim3(grid_for(tot)), dim3(256), 0, st, (const T*)w.A[0], w.Y,
                          N, Np, sep);
       // make Q unitary: Newton-Schulz Q <- Q (3I - Q^H Q)/2 until max|Q^H Q - I|
-      // was below 1e-8 before the last step (the error then squares to < 1e-16)
+      // was below 1e-8 before the last step (the error then squares to < 1e-16);
+      // the deviations (slots red[2..5]) are read after steps 2 and 4
       bool unitary = false;
+      unsigned long long* nsdev = reinterpret_cast<unsigned long long*>(w.red + 2);
       for (int ns = 0; ns < 4 && !unitary; ++ns) {
         gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), w.Y, Np, w.Y, Np, T(0), w.A[1], Np, tl);
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, nsdev + ns);
         hipLaunchKernelGGL(k_ns_step<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0,
-                           st, w.A[1], Np, w.off);
+                           st, w.A[1], Np, nsdev + ns);
         gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, w.A[1], Np, T(0), w.V[1], Np, tl);
         std::swap(w.Y, w.V[1]);
-        unsigned long long db = 0;
-        DMRG_HIP(hipMemcpyAsync(&db, w.off, sizeof(db), hipMemcpyDeviceToHost, st));
-        DMRG_HIP(hipStreamSynchronize(st));
-        double dv;
-        std::memcpy(&dv, &db, sizeof(dv));
-        unitary = dv <= 1e-8;
+        if (ns % 2 == 1) {
+          double dv[4];
+          DMRG_HIP(hipMemcpyAsync(dv, w.red + 2, (ns + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                  st));
+          DMRG_HIP(hipStreamSynchronize(st));
+          if (!(dv[ns] == dv[ns])) break;
+          unitary = dv[ns] <= 1e-8;
+        }
       }
       if (!unitary) break;  // leave it to Jacobi (A, V unchanged)
       std::swap(w.Y, w.V[1]);  // Q2 -> V[1]
@@ -816,8 +931,11 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
       std::swap(w.V[0], w.Y);
       hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[0], w.V[0], N, Np, 0);
       DMRG_LAUNCH_CHECK();
+      ++cnt.oa;
     }
   }
+  bool bip_ok = true;  // bipartite sweeps allowed (revoked once one stalls)
+  int n_bip = 0;
   for (int sw = 0; sw < prm.max_sweeps && !done; ++sw) {
     // sort the diagonal, permute rho and V into the other buffer
     hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N,
@@ -825,14 +943,29 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, i
     hipLaunchKernelGGL(k_permute<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[cur], w.V[cur],
                        w.A[1 - cur], w.V[1 - cur], w.perm, Np);
     cur = 1 - cur;
-    for (int r = 0; r < nb - 1; ++r) {
-      hipLaunchKernelGGL(k_pair_eig<T>, dim3(np), dim3(NTA), 0, st, w.A[cur], Np, nb, r, w.Q,
+    // From a warm start already close to the split, rotate only the couplings
+    // between the kept group (the first keep/JB blocks after the sort) and the
+    // rest: max(ka, nb-ka) rounds instead of nb-1.  From a far start that
+    // schedule need not converge (the kept group itself is still mixing), and
+    // near-degenerate rows at the cut can trade places between sweeps, which
+    // turns couplings inside a group into cross couplings; so bipartite sweeps
+    // run only while each cuts the measure tenfold (at most 4 per split), and
+    // the full round robin finishes the job.
+    const int ka = (bip_ok && keep > 0 && keep % JB == 0 && keep < Np && v_given &&
+                    last_off <= 1e-4 * fro) ? keep / JB : 0;
+    const double off_before = last_off;
+    const int rounds = sched_rounds(nb, ka);
+    for (int r = 0; r < rounds; ++r) {
+      hipLaunchKernelGGL(k_pair_eig<T>, dim3(np), dim3(NTA), 0, st, w.A[cur], Np, nb, ka, r, w.Q,
                          w.qflag, (const double*)w.scal, rot_tol, inner_max);
       hipLaunchKernelGGL(k_pair_apply<T>, dim3(noff + nvt), dim3(NT), 0, st, w.A[cur], w.V[cur], Np,
-                         nb, r, (const T*)w.Q, (const int*)w.qflag);
+                         nb, ka, r, (const T*)w.Q, (const int*)w.qflag);
     }
     double off = offmax(w.A[cur]);
     if (off <= prm.tol * fro) off = accept_keep(w.A[cur], off);
+    last_off = off;
+    if (ka > 0) ++cnt.bip;
+    if (ka > 0 && (++n_bip >= 4 || off > 0.1 * off_before)) bip_ok = false;
     sweeps = sw + 1;
     off_rel = off / fro;
     if (off <= prm.tol * fro) { done = true; break; }
@@ -862,6 +995,7 @@ void SvdWork<T>::alloc(int cap_n) {
   DMRG_HIP(hipMalloc(&perm, cap * sizeof(int)));
   DMRG_HIP(hipMalloc(&rank, cap * sizeof(int)));
   DMRG_HIP(hipMalloc(&gdiag, cap * sizeof(double)));
+  DMRG_HIP(hipMalloc(&red, 2 * NRED * sizeof(double)));
   DMRG_HIP(hipMalloc(&sigma, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&resid, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&scal, 4 * sizeof(double)));
@@ -876,7 +1010,7 @@ void SvdWork<T>::release() {
     if (V[i]) (void)hipFree(V[i]);
     A[i] = V[i] = nullptr;
   }
-  void* ps[] = {Y, Q, qflag, perm, rank, gdiag, sigma, resid, scal, off, newk};
+  void* ps[] = {Y, Q, qflag, perm, rank, gdiag, red, sigma, resid, scal, off, newk};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   Y = nullptr;
@@ -885,6 +1019,7 @@ void SvdWork<T>::release() {
   perm = nullptr;
   rank = nullptr;
   gdiag = nullptr;
+  red = nullptr;
   sigma = nullptr;
   resid = nullptr;
   scal = nullptr;
@@ -921,8 +1056,12 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
   SvdResult res;
   res.N = N;
   const int keep = (prm.subspace && k < N) ? k : 0;
-  const int cur = jacobi<T>(st, w, N, Np, prm, res.sweeps, res.off, warm != nullptr, keep, cutoff,
+  JacobiCount jc;
+  const int cur = jacobi<T>(st, w, N, Np, prm, jc, res.off, warm != nullptr, keep, cutoff,
                             rel_cutoff);
+  res.sweeps = jc.sweeps;
+  res.bip_sweeps = jc.bip;
+  res.oa_steps = jc.oa;
   if (basis_out)
     hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)N * N)), dim3(256), 0, st,
                        (const T*)w.V[cur], Np, (const int*)w.perm, N, N, 0, basis_out, N);
@@ -980,7 +1119,7 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
       hipLaunchKernelGGL(k_vecnorm<T>, dim3(k), dim3(256), 0, st, (const T*)R, m, len, 0, w.sigma);
     }
   }
-  hipLaunchKernelGGL(k_rank, dim3(1), dim3(1), 0, st, (const double*)w.sigma,
+  hipLaunchKernelGGL(k_rank, dim3(1), dim3(256), 0, st, (const double*)w.sigma,
                      (const double*)w.gdiag, k, cutoff, rel_cutoff, w.scal, w.newk);
   DMRG_LAUNCH_CHECK();
   int nk = 0;
@@ -996,6 +1135,8 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
     SvdResult r2 = svd_split<T>(st, w, A, m, n, lda, kmax, cutoff, rel_cutoff, scale_left, left,
                                 ldl, right, sigma_out, full, warm, basis_out);
     r2.sweeps += res.sweeps;
+    r2.bip_sweeps += res.bip_sweeps;
+    r2.oa_steps += res.oa_steps;
     return r2;
   }
   res.nk = nk;
@@ -1020,9 +1161,10 @@ int heig(hipStream_t st, SvdWork<T>& w, const T* H, int N, int ldh, double* eval
   DMRG_REQUIRE(Np <= w.cap, "heig: matrix exceeds workspace");
   DMRG_HIP(hipMemcpy2DAsync(w.A[0], (size_t)Np * sizeof(T), H, (size_t)ldh * sizeof(T),
                             (size_t)N * sizeof(T), N, hipMemcpyDeviceToDevice, st));
-  int sweeps = 0;
+  JacobiCount jc;
   double off = 0.0;
-  const int cur = jacobi<T>(st, w, N, Np, prm, sweeps, off);
+  const int cur = jacobi<T>(st, w, N, Np, prm, jc, off);
+  const int sweeps = jc.sweeps;
   hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, (const T*)w.A[cur], Np, nullptr, N, N,
                      w.perm, evals);
   hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)N * N)), dim3(256), 0, st,

@@ -46,6 +46,7 @@ struct SvdWork {
   int* perm = nullptr;           // sort permutation
   int* rank = nullptr;           // inverse of perm (position of each index)
   double* gdiag = nullptr;       // squared norms of the k product vectors
+  double* red = nullptr;         // reduction partials / small results (512 doubles)
   double* sigma = nullptr;       // singular values in output order
   double* resid = nullptr;       // first-pass QR residuals
   double* scal = nullptr;        // [0] |rho|_F, [1] discarded weight, [2] trace(rho)
@@ -71,7 +72,9 @@ struct SvdParams {
 
 struct SvdResult {
   int N;             // dimension of the density matrix that was diagonalised
-  int sweeps;        // outer Jacobi sweeps
+  int sweeps;        // outer Jacobi sweeps (all schedules)
+  int bip_sweeps;    // of which kept-vs-rest (bipartite) sweeps
+  int oa_steps;      // first-order all-pairs steps before the sweeps
   double off;        // final max |rho_ij| / |rho|_F
   int nk;            // kept rank
   double discarded;  // sum of the discarded sigma^2 (trace - kept)

@@ -62,7 +62,8 @@ class Stats(C.Structure):
                 ("lanczos_calls", C.c_longlong), ("svd_calls", C.c_longlong),
                 ("svd_sweeps", C.c_longlong), ("sweeps_done", C.c_int),
                 ("max_discarded", C.c_double), ("env_build_s", C.c_double),
-                ("sweeps_s_total", C.c_double), ("svd_warm_starts", C.c_longlong)]
+                ("sweeps_s_total", C.c_double), ("svd_warm_starts", C.c_longlong),
+                ("svd_bip_sweeps", C.c_longlong), ("svd_oa_steps", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

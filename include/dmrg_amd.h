@@ -88,6 +88,8 @@ typedef struct dmrg_stats {
   double env_build_s;    /* wall seconds of the last environment build      */
   double sweeps_s_total; /* wall seconds of all sweeps (environment build excluded) */
   long long svd_warm_starts; /* splits that started from a previous sweep's basis */
+  long long svd_bip_sweeps;  /* Jacobi sweeps that rotated only kept-vs-discarded pairs */
+  long long svd_oa_steps;    /* first-order all-pairs steps before the Jacobi sweeps */
 } dmrg_stats;
 
 typedef struct dmrg_engine* dmrg_handle;
