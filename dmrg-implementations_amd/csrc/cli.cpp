@@ -6,6 +6,9 @@
 //   dmrg2_gpu_amd L chi_max n_sweeps [--josephson --nmax N --ej X --ec X --phi X]
 //                 [--tfim --hfield X] [--quiet]
 //   extras: [--single-site] [--fp32] [--krylov K] [--seed S] [--random] [--neel]
+//           [--mpo FILE.bin [--mps FILE.bin]]: Hamiltonian (and initial state)
+//           from the reference's benchmark files (benchmarks/lib/data_loader.py:24-88)
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -94,6 +97,82 @@ static std::vector<std::vector<cplx>> josephson(int L, int n_max, double EJ, dou
   return W;
 }
 
+// --- reference .bin files ---------------------------------------------------
+// int64 L, int64 bonds[L+1], int64 phys[L], then per site int64 shape[nd] and
+// complex128 entries in C order; MPS (Dl, d, Dr), MPO (Dl, up, down, Dr) with
+// quimb's bond convention (bulk from D-1 on the left to 0 on the right, 1-wide
+// outer edge bonds).
+struct BinTensor {
+  std::vector<long long> shape;
+  std::vector<cplx> data;
+};
+
+static bool read_bin(const char* path, int nd, std::vector<BinTensor>& out, std::string& err) {
+  FILE* f = fopen(path, "rb");
+  if (!f) { err = std::string("cannot open ") + path; return false; }
+  auto rd = [&](void* p, size_t sz, size_t n) { return fread(p, sz, n, f) == n; };
+  long long L = 0;
+  bool ok = rd(&L, 8, 1) && L >= 1 && L <= (1 << 20);
+  std::vector<long long> bonds, phys;
+  if (ok) {
+    bonds.resize(L + 1);
+    phys.resize(L);
+    ok = rd(bonds.data(), 8, L + 1) && rd(phys.data(), 8, L);
+  }
+  for (long long i = 0; ok && i < L; ++i) {
+    BinTensor t;
+    t.shape.resize(nd);
+    ok = rd(t.shape.data(), 8, nd);
+    if (!ok) break;
+    ok = t.shape[0] == bonds[i] && t.shape[nd - 1] == bonds[i + 1] && t.shape[1] == phys[i] &&
+         (nd == 3 || t.shape[2] == phys[i]);
+    long long n = 1;
+    for (long long x : t.shape) {
+      if (x < 1 || x > (1 << 20)) ok = false;
+      n *= x;
+    }
+    if (!ok || n > (1ll << 31)) { ok = false; break; }
+    t.data.resize(n);
+    ok = rd(t.data.data(), sizeof(cplx), n);
+    out.push_back(std::move(t));
+  }
+  if (ok && fgetc(f) != EOF) ok = false;
+  fclose(f);
+  if (!ok) err = std::string(path) + ": not a valid .bin tensor file of this kind";
+  return ok;
+}
+
+// engine W[w + s D + sp D d + wp D d d] from (Dl, up, down, Dr): bond index
+// flipped, the 1-wide outer edge bond placed on the boundary row / column,
+// (up, down) -> (sp, s).
+static std::vector<std::vector<cplx>> mpo_from_bin(const std::vector<BinTensor>& ts, int& D, int& d) {
+  const int L = (int)ts.size();
+  D = 0;
+  for (const auto& t : ts) D = std::max(D, (int)std::max(t.shape[0], t.shape[3]));
+  d = (int)ts[0].shape[1];
+  std::vector<std::vector<cplx>> W(L, std::vector<cplx>((size_t)D * d * d * D));
+  for (int i = 0; i < L; ++i) {
+    const auto& t = ts[i];
+    const int dl = (int)t.shape[0], dr = (int)t.shape[3];
+    for (int l = 0; l < dl; ++l)
+      for (int u = 0; u < d; ++u)
+        for (int dn = 0; dn < d; ++dn)
+          for (int r = 0; r < dr; ++r) {
+            const int w = (i == 0 && dl == 1) ? 0 : D - 1 - l;
+            const int wp = (i == L - 1 && dr == 1) ? D - 1 : D - 1 - r;
+            W[i][w + dn * D + u * D * d + (size_t)wp * D * d * d] = t.data[((l * d + u) * d + dn) * dr + r];
+          }
+  }
+  return W;
+}
+
+static bool all_real(const std::vector<std::vector<cplx>>& v) {
+  for (const auto& a : v)
+    for (const auto& x : a)
+      if (x.imag() != 0.0) return false;
+  return true;
+}
+
 static int check(int rc, const char* what) {
   if (rc != DMRG_OK) {
     fprintf(stderr, "ERROR: %s: %s\n", what, dmrg_last_error());
@@ -107,6 +186,8 @@ int main(int argc, char** argv) {
   bool jj = false, tf = false, quiet = false;
   double EJ = 1.0, EC = 0.5, phi = M_PI / 4, hfield = 1.0;
   unsigned long long seed = 1;
+  const char* mpo_file = nullptr;
+  const char* mps_file = nullptr;
   int pos = 0;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -128,6 +209,8 @@ int main(int argc, char** argv) {
     else if (a == "--hfield") hfield = atof(next());
     else if (a == "--krylov") krylov = atoi(next());
     else if (a == "--seed") seed = strtoull(next(), nullptr, 10);
+    else if (a == "--mpo") mpo_file = next();
+    else if (a == "--mps") mps_file = next();
     else if (a.size() && a[0] == '-') { fprintf(stderr, "Unknown flag: %s\n", a.c_str()); return 1; }
     else if (pos == 0) { L = atoi(a.c_str()); ++pos; }
     else if (pos == 1) { chi = atoi(a.c_str()); ++pos; }
@@ -140,6 +223,52 @@ int main(int argc, char** argv) {
   dmrg_handle h = nullptr;
   int variant = single ? DMRG_SINGLE_SITE : DMRG_TWO_SITE;
   double energy = 0.0;
+  if (mps_file && !mpo_file) { fprintf(stderr, "ERROR: --mps needs --mpo\n"); return 1; }
+  if (mpo_file) {
+    std::vector<BinTensor> tw, ta;
+    std::string err;
+    if (!read_bin(mpo_file, 4, tw, err) || (mps_file && !read_bin(mps_file, 3, ta, err))) {
+      fprintf(stderr, "ERROR: %s\n", err.c_str());
+      return 1;
+    }
+    int D = 0, d = 0;
+    std::vector<std::vector<cplx>> W = mpo_from_bin(tw, D, d);
+    const int Lf = (int)W.size();
+    if (pos > 0 && L != Lf) { fprintf(stderr, "ERROR: L=%d but %s has %d sites\n", L, mpo_file, Lf); return 1; }
+    if (mps_file && (int)ta.size() != Lf) { fprintf(stderr, "ERROR: MPS and MPO site counts differ\n"); return 1; }
+    std::vector<std::vector<cplx>> A(ta.size());
+    for (size_t i = 0; i < ta.size(); ++i) {  // (Dl, d, Dr) C order -> column-major
+      const long long cl = ta[i].shape[0], dd = ta[i].shape[1], cr = ta[i].shape[2];
+      if (dd != d) { fprintf(stderr, "ERROR: MPS and MPO physical dimensions differ\n"); return 1; }
+      A[i].resize(cl * dd * cr);
+      for (long long a = 0; a < cl; ++a)
+        for (long long q = 0; q < dd; ++q)
+          for (long long b = 0; b < cr; ++b) A[i][a + q * cl + b * cl * dd] = ta[i].data[(a * dd + q) * cr + b];
+    }
+    const bool real = all_real(W) && all_real(A);
+    check(dmrg_create(Lf, d, chi, D, 1e-10, real ? DMRG_DTYPE_F64 : DMRG_DTYPE_C128, variant, &o, &h),
+          "create");
+    std::vector<std::vector<double>> Wr, Ar;
+    std::vector<const void*> p(Lf);
+    if (real) {
+      for (auto& w : W) { Wr.emplace_back(w.size()); for (size_t k = 0; k < w.size(); ++k) Wr.back()[k] = w[k].real(); }
+      for (auto& a : A) { Ar.emplace_back(a.size()); for (size_t k = 0; k < a.size(); ++k) Ar.back()[k] = a[k].real(); }
+      for (int i = 0; i < Lf; ++i) p[i] = Wr[i].data();
+    } else {
+      for (int i = 0; i < Lf; ++i) p[i] = W[i].data();
+    }
+    check(dmrg_set_mpo(h, p.data()), "set_mpo");
+    if (mps_file) {
+      for (int i = 0; i < Lf; ++i)
+        check(dmrg_set_mps(h, i, real ? (const void*)Ar[i].data() : (const void*)A[i].data(),
+                           (int)ta[i].shape[0], (int)ta[i].shape[2]), "set_mps");
+    } else {
+      check(dmrg_init_mps_random(h, seed, 0.1), "init");
+    }
+    check(dmrg_run(h, sweeps, quiet ? 1 : 0, &energy), "run");
+    dmrg_destroy(h);
+    return 0;
+  }
   if (jj) {
     int d = 2 * n_max + 1;
     auto W = josephson(L, n_max, EJ, EC, phi);

@@ -885,11 +885,14 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
     // (all-pairs, GEMM-only) corrections before any Jacobi sweep.  A pair is
     // corrected when its gap exceeds sep |A_ij| (the first-order step then
     // leaves O(|A_ij|/sep)); the subspace criterion tolerates the rougher sep.
-    // Steps go on while each one cuts the measure at least tenfold.
+    // Steps go on while each one cuts the measure at least tenfold.  Pairs
+    // near the cut whose gap is only 30-100 times their coupling stall the
+    // step at sep = 1e2; the subspace criterion then retries with sep = 30
+    // (first-order error |A_ij|/30 per step) before handing over to Jacobi.
     const GemmTile tl = gemm_pick_tile(Np, Np, Np, is_cplx<T>::value);
-    const double sep = keep > 0 ? 1e2 : 1e4;
+    double sep = keep > 0 ? 1e2 : 1e4;
     const double gate = keep > 0 ? 1e-4 : 1e-6;
-    const int max_steps = keep > 0 ? 3 : 2;
+    const int max_steps = keep > 0 ? 5 : 2;
     for (int it = 0; it <= max_steps && !done; ++it) {
       double o = offmax(w.A[0]);
       if (o <= prm.tol * fro) o = accept_keep(w.A[0], o);
@@ -898,7 +901,10 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
       off_rel = o / fro;
       if (o <= prm.tol * fro) { done = true; break; }
       if (it == max_steps || o > gate * fro) break;  // not perturbative: Jacobi
-      if (it > 0 && o > 0.1 * prev) break;           // stalled (clusters): Jacobi
+      if (it > 0 && o > 0.1 * prev) {                // stalled (near-degenerate pairs)
+        if (keep > 0 && sep > 30.0) sep = 30.0;
+        else break;                                  // Jacobi
+      }
       hipLaunchKernelGGL(k_oa_q<T>, dim3(grid_for(tot)), dim3(256), 0, st, (const T*)w.A[0], w.Y,
                          N, Np, sep);
       // make Q unitary: Newton-Schulz Q <- Q (3I - Q^H Q)/2 until max|Q^H Q - I|

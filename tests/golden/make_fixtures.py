@@ -5,6 +5,8 @@
 Reads DATA files of the reference checkout (never its code):
   benchmark_data/regular/heisenberg/L12_D20/{mpo.npz, initial_mps.npz, golden_results.json}
   benchmark_data/regular/josephson/L20_D50_nmax2/{mpo.npz, golden_results.json}
+  benchmarks/data/{heisenberg_L12_mpo, heisenberg_L12_chi10_mps,
+                   josephson_L8_n2_mpo, josephson_L8_n2_chi10_mps}.bin (copied as they are)
 The npz files are opened with allow_pickle=False; their pickled 'metadata'
 entry is skipped.  Tensors are converted from the quimb layout of those files
 (MPO bulk [l, r, up, down], 3-D edges, left boundary at l = D-1, right boundary
@@ -80,6 +82,12 @@ def main(ref):
     W = quimb_mpo_to_engine(z, 20)
     np.savez(os.path.join(HERE, "josephson_L20_D50_nmax2.npz"), W=W,
              E_dmrg1=g["quimb_dmrg1"]["energy"], E_dmrg2=g["quimb_dmrg2"]["energy"], chi=50)
+    # .bin files of benchmarks/data (seed 42): copied byte for byte
+    src = os.path.join(ref, "benchmarks", "data")
+    for name in ("heisenberg_L12_mpo.bin", "heisenberg_L12_chi10_mps.bin",
+                 "josephson_L8_n2_mpo.bin", "josephson_L8_n2_chi10_mps.bin"):
+        with open(os.path.join(src, name), "rb") as fi, open(os.path.join(HERE, name), "wb") as fo:
+            fo.write(fi.read())
     print("wrote fixtures to", HERE)
 
 
