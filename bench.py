@@ -1,6 +1,6 @@
 """Headline benchmark: DMRG sweep time and H_eff TFLOP/s, mixed vs FP64.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c2|small]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c2|c3|c5|small]
 
 Workload (default "c4"): spin-1/2 Heisenberg XXX chain, L=128, open BC, J=1,
 d=2, MPO bond D=5, chi_max=512, two-site DMRG, Lanczos Krylov cap 20,
@@ -11,12 +11,18 @@ A step is one full sweep (left-to-right plus right-to-left).  For each of the
 FP64 path and the mixed path (FP32 MFMA H_eff matvecs; FP64 environments,
 Lanczos and SVD) the engine runs W warm-up sweeps from the seeded state, then
 exactly K timed sweeps, one per engine call (the environment build before each
-call is outside the sweep timer, as in the reference).  `value` is the mixed
-path's mean seconds per sweep; the per-sweep times and their median are in
-`modes`.  The
-roofline entry is the H_eff matvec (three MFMA GEMM launches), timed with
-HIP events on the engine's stream.  |dE| compares the two paths after the
-same number of sweeps from the same state.
+call is outside the sweep timer, as in the reference).  The defaults, W=3 and
+K=9, make the 12-sweep C4 run.  `value` is the mixed path's mean seconds per
+timed sweep; every sweep's time (warm-up ones included), their mean over the
+whole run and the median of the timed ones are in `modes`.  The roofline entry is the H_eff matvec (two MFMA GEMM launches
+around the MPO-bond row kernel), timed with HIP events on the engine's
+stream.  |dE| compares the two paths after the same number of sweeps from the
+same state.
+
+Other workloads (not the headline line): c2 (Heisenberg L=64 chi=256), c3
+(complex Josephson array L=32 d=5 chi=256, FP64 only: the FP32 matvec is
+real-only), c5 (64 uneven random-field chains: sequential plain FP64 solves
+against dmrg_run_batch in FP64 and in the mixed mode; value = speedup).
 
 cpu_baseline: the reference CPU path (cpu/pdmrg tensordot H_eff + ARPACK
 eigsh + numpy SVD, restated in oracle/pdmrg_serial.py) timed on this host for
@@ -36,14 +42,25 @@ sys.path.insert(0, ROOT)
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
 
 WORKLOADS = {
-    "c4": dict(L=128, chi=512, krylov=20, seed=128),
-    "c2": dict(L=64, chi=256, krylov=20, seed=64),
-    "small": dict(L=32, chi=64, krylov=20, seed=32),
+    # the headline: Heisenberg XXX L=128 chi=512, FP64 vs mixed
+    "c4": dict(model="heisenberg", L=128, chi=512, krylov=20, seed=128),
+    # parity-test configurations, measured on request
+    "c2": dict(model="heisenberg", L=64, chi=256, krylov=20, seed=64),
+    "c3": dict(model="josephson", L=32, chi=256, krylov=20, seed=32, d=5, D=4, dtype="c128",
+               n_max=2, E_J=2.0, E_C=1.0, phi=3.141592653589793 / 3),
+    "c5": dict(model="batch", chains=64, seed=5, krylov=20),
+    "small": dict(model="heisenberg", L=32, chi=64, krylov=20, seed=32),
 }
 
 
-def heisenberg_mpo(L):
-    """Same MPO as oracle.models.heisenberg_mpo (kept here so the GPU legs do
+def _block(W, w, wp, op):
+    """W[w, s, sp, wp] = op[sp, s] (op as a <bra|op|ket> matrix)."""
+    W[w, :, :, wp] = op.T
+
+
+def heisenberg_mpo(L, h=None):
+    """Spin-1/2 XXX chain, D=5, with an optional field sum_i h_i Sz_i (same MPO
+    as oracle.models.random_field_heisenberg_mpo; kept here so the GPU legs do
     not import the oracle)."""
     import numpy as np
     Sp = np.array([[0.0, 1.0], [0.0, 0.0]])
@@ -53,15 +70,52 @@ def heisenberg_mpo(L):
     Ws = []
     for i in range(L):
         W = np.zeros((5, 2, 2, 5))
-        blocks = []
         if i != L - 1:
-            blocks += [(0, 0, I2), (0, 1, Sp), (0, 2, Sm), (0, 3, Sz)]
+            for wp, op in ((0, I2), (1, Sp), (2, Sm), (3, Sz)):
+                _block(W, 0, wp, op)
         if i != 0:
-            blocks += [(1, 4, 0.5 * Sm), (2, 4, 0.5 * Sp), (3, 4, Sz), (4, 4, I2)]
-        for w, wp, op in blocks:
-            W[w, :, :, wp] = op.T
+            for w, op in ((1, 0.5 * Sm), (2, 0.5 * Sp), (3, Sz), (4, I2)):
+                _block(W, w, 4, op)
+        if h is not None and h[i] != 0.0:
+            _block(W, 0, 4, h[i] * Sz)
         Ws.append(W)
     return Ws
+
+
+def josephson_mpo(L, n_max, E_J, E_C, phi):
+    """Josephson-junction array, complex, D=4 (same MPO as
+    oracle.models.josephson_mpo): E_C n^2 on site, -E_J/2 e^{i phi} on
+    e^{i phi_i} e^{-i phi_{i+1}} plus its conjugate."""
+    import numpy as np
+    d = 2 * n_max + 1
+    q = np.arange(d) - n_max
+    onsite = np.diag(E_C * q * q).astype(complex)
+    up = np.zeros((d, d), complex)
+    for k in range(d - 1):
+        up[k + 1, k] = 1.0
+    down = up.conj().T
+    a = -0.5 * E_J * np.exp(1j * phi)
+    eye = np.eye(d, dtype=complex)
+    Ws = []
+    for i in range(L):
+        W = np.zeros((4, d, d, 4), complex)
+        if i != L - 1:
+            _block(W, 0, 0, eye)
+            _block(W, 0, 1, a * up)
+            _block(W, 0, 2, np.conj(a) * down)
+        _block(W, 0, 3, onsite)
+        if i != 0:
+            _block(W, 1, 3, down)
+            _block(W, 2, 3, up)
+            _block(W, 3, 3, eye)
+        Ws.append(W)
+    return Ws
+
+
+def workload_mpo(wl):
+    if wl["model"] == "josephson":
+        return josephson_mpo(wl["L"], wl["n_max"], wl["E_J"], wl["E_C"], wl["phi"])
+    return heisenberg_mpo(wl["L"])
 
 
 # Jacobi stop of the truncation split (max kept/discarded coupling relative to
@@ -71,13 +125,16 @@ SVD_TOL = {"fp64": 1e-14, "fp32": 1e-14}
 
 
 def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
-    Ws = heisenberg_mpo(wl["L"])
-    eng = dmrg_amd.DMRG(wl["L"], 2, wl["chi"], 5, tol=0.0, krylov_max=wl["krylov"],
+    Ws = workload_mpo(wl)
+    eng = dmrg_amd.DMRG(wl["L"], wl.get("d", 2), wl["chi"], wl.get("D", 5),
+                        dtype=wl.get("dtype", "f64"), tol=0.0, krylov_max=wl["krylov"],
                         precision=precision, svd_tol=svd_tol)
     eng.set_mpo(Ws)
     eng.initialize_mps_random(seed=wl["seed"], scale=0.1)
-    if warmup > 0:
-        eng.run(warmup)
+    warm_sweeps = []
+    for _ in range(warmup):  # one sweep per call, timed apart (reported, not in `value`)
+        eng.run(1)
+        warm_sweeps.append(eng.stats()["sweep_s_last"])
     eng.reset_stats()
     dmrg_amd.synchronize()
     # one sweep per call: each call rebuilds the environments (outside the
@@ -99,6 +156,8 @@ def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
                svd_warm_starts=st["svd_warm_starts"], svd_bip_sweeps=st["svd_bip_sweeps"],
                svd_oa_steps=st["svd_oa_steps"], sweep_s_each=per_sweep,
                sweep_s_median=sorted(per_sweep)[len(per_sweep) // 2] if per_sweep else 0.0,
+               warmup_sweep_s=warm_sweeps,
+               sweep_s_mean_all=(sum(warm_sweeps) + sum(per_sweep)) / max(len(warm_sweeps) + len(per_sweep), 1),
                ms_per_matvec=st["heff_ms"] / max(st["matvecs"], 1))
     return eng, res
 
@@ -109,7 +168,7 @@ def cpu_baseline(eng, wl, nbonds):
     import numpy as np
     from oracle import pdmrg_serial as ps
     L, chi = wl["L"], wl["chi"]
-    Ws = [ps.to_pdmrg_mpo(W) for W in heisenberg_mpo(L)]
+    Ws = [ps.to_pdmrg_mpo(W) for W in workload_mpo(wl)]
     mps = eng.get_mps()
     c0 = L // 2 - nbonds // 2
     L_envs = {c0: ps.to_pdmrg_env(eng.get_env(True, c0))}
@@ -128,15 +187,89 @@ def cpu_baseline(eng, wl, nbonds):
                        "scaled by the 2*(L-1) bond updates of a sweep" % (nbonds, chi, per_bond))
 
 
+METRIC_C4 = "sweep s + H_eff TFLOP/s at L=128 \u03c7=512, mixed vs FP64; |\u0394E| vs FP64"
+
+
+def batch_chains(wl):
+    """64 random-field Heisenberg chains: L in {16,24,32,48,64}, chi_max in
+    {32,64,128,256}, disorder W in {0.5,1,2,5}, h_i ~ U[-W, W] (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(wl["seed"])
+    out = []
+    for _ in range(wl["chains"]):
+        L = int(rng.choice([16, 24, 32, 48, 64]))
+        chi = int(rng.choice([32, 64, 128, 256]))
+        W = float(rng.choice([0.5, 1.0, 2.0, 5.0]))
+        out.append((L, chi, W, rng.uniform(-W, W, L)))
+    return out
+
+
+def run_batch_bench(dmrg_amd, wl, a):
+    """Sequential plain FP64 solves of every chain, then the same chains run
+    together by dmrg_run_batch (FP64, and FP32-matvec mode); each batched
+    energy is compared with its chain's own FP64 solve."""
+    import numpy as np
+    chains = batch_chains(wl)
+    sweeps = a.batch_sweeps
+    ref, t_seq = [], 0.0
+    for i, (L, chi, W, h) in enumerate(chains):
+        e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=wl["krylov"])
+        e.set_mpo(heisenberg_mpo(L, h))
+        e.initialize_mps_random(seed=1000 + i, scale=0.1)
+        dmrg_amd.synchronize()
+        t0 = time.perf_counter()
+        ref.append(e.run(sweeps))
+        dmrg_amd.synchronize()
+        t_seq += time.perf_counter() - t0
+        e.close()
+    print("[bench] c5 sequential fp64: %.2f s for %d chains" % (t_seq, len(chains)),
+          file=sys.stderr, flush=True)
+    out_modes = {}
+    for m, prec in (("fp64", dmrg_amd.PREC_FP64), ("fp32", dmrg_amd.PREC_FP32)):
+        engines = []
+        for i, (L, chi, W, h) in enumerate(chains):
+            e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=wl["krylov"], precision=prec)
+            e.set_mpo(heisenberg_mpo(L, h))
+            e.initialize_mps_random(seed=1000 + i, scale=0.1)
+            engines.append(e)
+        dmrg_amd.synchronize()
+        t0 = time.perf_counter()
+        E, T = dmrg_amd.run_batch(engines, sweeps, threads=a.batch_threads)
+        dmrg_amd.synchronize()
+        t_b = time.perf_counter() - t0
+        for e in engines:
+            e.close()
+        rel = [abs(E[i] - ref[i]) / abs(ref[i]) for i in range(len(chains))]
+        out_modes[m] = dict(batch_s=t_b, speedup_vs_sequential=t_seq / t_b, max_dE_rel=max(rel),
+                            median_dE_rel=float(np.median(rel)))
+        print("[bench] c5 batch %s: %s" % (m, json.dumps(out_modes[m])), file=sys.stderr, flush=True)
+    head = out_modes["fp32"]
+    print(json.dumps({
+        "metric": "64-chain uneven batch: sequential plain FP64 time / batched time (mixed); "
+                  "max |\u0394E|/|E| vs each chain's own FP64 solve",
+        "value": head["speedup_vs_sequential"], "unit": "x", "n_gpus": 1, "steps": sweeps,
+        "warmup": 0, "ms_per_step": head["batch_s"] * 1e3 / sweeps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 H_eff MFMA / f64",
+        "data": "synthetic (seeded random-field chains and initial MPS)",
+        "config": {"workload": "64 random-field Heisenberg chains, L in {16,24,32,48,64}, chi_max in "
+                               "{32,64,128,256}, W in {0.5,1,2,5}, seed %d, %d sweeps each, %d host "
+                               "threads" % (wl["seed"], sweeps, a.batch_threads)},
+        "sequential_fp64_s": t_seq, "modes": out_modes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=2)
+    # C4 is a 12-sweep run: 3 sweeps from the random start (untimed in `value`),
+    # then 9 timed sweeps
+    ap.add_argument("--steps", type=int, default=9)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-bonds", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--batch-sweeps", type=int, default=6, help="c5: sweeps per chain")
+    ap.add_argument("--batch-threads", type=int, default=8, help="c5: host threads (chains in flight)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="before the runs, copy MIB MiB device-to-device 3 times (the known-size "
                          "copy kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -156,7 +289,12 @@ def main():
         dmrg_amd.synchronize()
         src.free()
         dst.free()
+    if wl["model"] == "batch":
+        run_batch_bench(dmrg_amd, wl, a)
+        return
     modes = a.modes.split(",")
+    if wl.get("dtype") == "c128":
+        modes = [m for m in modes if m == "fp64"] or ["fp64"]  # the FP32 matvec is real-only
     res, engines = {}, {}
     for m in modes:
         prec = dmrg_amd.PREC_FP64 if m == "fp64" else dmrg_amd.PREC_FP32
@@ -175,8 +313,20 @@ def main():
         e.close()
     r = res[head]
     peak = PEAK["fp32" if head == "fp32" else "fp64"]
+    if a.workload == "c4":
+        metric = METRIC_C4
+    else:
+        metric = "sweep s + H_eff TFLOP/s at %s L=%d \u03c7=%d" % (wl["model"], wl["L"], wl["chi"])
+    if wl["model"] == "josephson":
+        desc = ("josephson_array L=%d d=%d D=4 chi_max=%d complex128 E_J=%g E_C=%g flux=pi/3 "
+                "two-site krylov=%d seed=%d" % (wl["L"], wl["d"], wl["chi"], wl["E_J"], wl["E_C"],
+                                                 wl["krylov"], wl["seed"]))
+    else:
+        desc = ("heisenberg_xxx L=%d chi_max=%d two-site D=5 d=2 krylov=%d seed=%d"
+                % (wl["L"], wl["chi"], wl["krylov"], wl["seed"]))
+    dtype = "f32 H_eff MFMA / f64" if head == "fp32" else ("c128" if wl.get("dtype") == "c128" else "f64")
     out = {
-        "metric": "sweep s + H_eff TFLOP/s at L=128 chi=512, mixed vs FP64; |dE| vs FP64",
+        "metric": metric,
         "value": r["sweep_s"],
         "unit": "s/sweep",
         "n_gpus": 1,
@@ -186,11 +336,9 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 H_eff MFMA / f64" if head == "fp32" else "f64",
+        "dtype": dtype,
         "data": "synthetic (seeded random initial MPS)",
-        "config": {"workload": "heisenberg_xxx L=%d chi_max=%d two-site D=5 d=2 krylov=%d seed=%d"
-                               % (wl["L"], wl["chi"], wl["krylov"], wl["seed"]),
-                   "L": wl["L"], "chi_max": wl["chi"], "krylov_max": wl["krylov"]},
+        "config": {"workload": desc, "L": wl["L"], "chi_max": wl["chi"], "krylov_max": wl["krylov"]},
         "roofline": {"bound": "mfma", "achieved": r["heff_tflops"], "peak": peak,
                      "unit": "TFLOP/s", "frac": r["heff_tflops"] / peak, "traffic": None},
         "cpu_baseline": cpu,
