@@ -187,6 +187,17 @@ def cpu_baseline(eng, wl, nbonds):
                        "scaled by the 2*(L-1) bond updates of a sweep" % (nbonds, chi, per_bond))
 
 
+def heff_traffic(workload, mode):
+    """HBM bytes per H_eff matvec at the C4 centre bond in the mixed mode, from
+    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (copy-calibrated);
+    None for other workloads and modes."""
+    path = os.path.join(ROOT, "profiles", "c4_heff_traffic.json")
+    if workload != "c4" or mode != "fp32" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["bytes_per_matvec"]
+
+
 METRIC_C4 = "sweep s + H_eff TFLOP/s at L=128 \u03c7=512, mixed vs FP64; |\u0394E| vs FP64"
 
 
@@ -340,7 +351,8 @@ def main():
         "data": "synthetic (seeded random initial MPS)",
         "config": {"workload": desc, "L": wl["L"], "chi_max": wl["chi"], "krylov_max": wl["krylov"]},
         "roofline": {"bound": "mfma", "achieved": r["heff_tflops"], "peak": peak,
-                     "unit": "TFLOP/s", "frac": r["heff_tflops"] / peak, "traffic": None},
+                     "unit": "TFLOP/s", "frac": r["heff_tflops"] / peak,
+                     "traffic": heff_traffic(a.workload, head)},
         "cpu_baseline": cpu,
         "modes": res,
     }
