@@ -8,8 +8,8 @@ seeded random initial MPS (splitmix64 seed 128, entries 0.1*U[-1,1)), bond
 dimensions pre-sized to min(2^min(i,L-i), chi_max) as the reference does.
 
 A step is one full sweep (left-to-right plus right-to-left).  For each of the
-FP64 path and the mixed path (FP32 MFMA H_eff matvecs; FP64 environments,
-Lanczos and SVD) the engine runs W warm-up sweeps from the seeded state, then
+FP64 path and the mixed paths (FP32 MFMA H_eff matvecs, and the same with split-BF16
+products; FP64 environments, Lanczos and SVD) the engine runs W warm-up sweeps from the seeded state, then
 exactly K timed sweeps, one per engine call (the environment build before each
 call is outside the sweep timer, as in the reference).  The defaults, W=3 and
 K=9, make the 12-sweep C4 run.  `value` is the mixed path's mean seconds per
@@ -39,7 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dmrg-implementations_amd"))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
+# MI355X dense MFMA TFLOP/s; split-BF16 counts the BF16 peak over its six partial products
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 6}
 
 WORKLOADS = {
     # the headline: Heisenberg XXX L=128 chi=512, FP64 vs mixed
@@ -121,7 +122,7 @@ def workload_mpo(wl):
 # Jacobi stop of the truncation split (max kept/discarded coupling relative to
 # |rho|_F), the same in both modes: a looser 1e-12 in the mixed mode measured
 # |dE|/|E| = 1.3e-10 against FP64 with no gain in split time.
-SVD_TOL = {"fp64": 1e-14, "fp32": 1e-14}
+SVD_TOL = {"fp64": 1e-14, "fp32": 1e-14, "bf16x3": 1e-14}
 
 
 def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
@@ -278,7 +279,7 @@ def main():
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-bonds", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--modes", default="fp64,fp32,bf16x3")
     ap.add_argument("--batch-sweeps", type=int, default=6, help="c5: sweeps per chain")
     ap.add_argument("--batch-threads", type=int, default=8, help="c5: host threads (chains in flight)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -308,7 +309,8 @@ def main():
         modes = [m for m in modes if m == "fp64"] or ["fp64"]  # the FP32 matvec is real-only
     res, engines = {}, {}
     for m in modes:
-        prec = dmrg_amd.PREC_FP64 if m == "fp64" else dmrg_amd.PREC_FP32
+        prec = {"fp64": dmrg_amd.PREC_FP64, "fp32": dmrg_amd.PREC_FP32,
+                "bf16x3": dmrg_amd.PREC_BF16X3}[m]
         print("[bench] %s: %d warm-up + %d timed sweeps, L=%d chi=%d" %
               (m, a.warmup, a.steps, wl["L"], wl["chi"]), file=sys.stderr, flush=True)
         eng, r = run_gpu(dmrg_amd, wl, prec, a.steps, a.warmup, SVD_TOL[m])
@@ -323,7 +325,7 @@ def main():
     for e in engines.values():
         e.close()
     r = res[head]
-    peak = PEAK["fp32" if head == "fp32" else "fp64"]
+    peak = PEAK[head]
     if a.workload == "c4":
         metric = METRIC_C4
     else:
@@ -361,6 +363,11 @@ def main():
         out["dE_abs"] = abs(e32 - e64)
         out["dE_rel"] = abs(e32 - e64) / abs(e64)
         out["speedup_mixed_vs_fp64"] = res["fp64"]["sweep_s"] / res["fp32"]["sweep_s"]
+    for m in res:
+        if m != "fp64" and "fp64" in res:
+            res[m]["dE_rel_vs_fp64"] = abs(res[m]["energy"] - res["fp64"]["energy"]) / abs(res["fp64"]["energy"])
+            res[m]["speedup_vs_fp64"] = res["fp64"]["sweep_s"] / res[m]["sweep_s"]
+        res[m]["heff_frac_of_peak"] = res[m]["heff_tflops"] / PEAK[m]
     print(json.dumps(out), flush=True)
 
 

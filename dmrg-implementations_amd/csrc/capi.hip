@@ -572,6 +572,23 @@ int dmrg_op_gemm(dmrg_ctx_handle c, void* stream, int dtype, int opA, int opB, i
   });
 }
 
+int dmrg_op_gemm_f32(dmrg_ctx_handle c, void* stream, int prec, int opA, int opB, int M, int N,
+                     int K, const float* A, int lda, const float* B, int ldb, float* Cm, int ldc) {
+  if (!c || !A || !B || !Cm) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
+  return guard([&] {
+    DMRG_REQUIRE(M >= 1 && N >= 1 && K >= 1, "bad shape");
+    DMRG_REQUIRE((opA == OP_N || opA == OP_T) && (opB == OP_N || opB == OP_T), "op must be N or T");
+    DMRG_REQUIRE(ldc >= M, "ldc < M");
+    DMRG_REQUIRE(lda >= (op_trans(opA) ? K : M) && ldb >= (op_trans(opB) ? N : K), "bad ld");
+    DMRG_REQUIRE(prec == DMRG_PREC_FP32 || prec == DMRG_PREC_BF16X3, "prec must be FP32 or BF16X3");
+    if (prec == DMRG_PREC_BF16X3)
+      gemm_f32_split_bf16((hipStream_t)stream, opA, opB, M, N, K, A, lda, B, ldb, Cm, ldc);
+    else
+      gemm<float>((hipStream_t)stream, opA, opB, M, N, K, 1.0f, A, lda, B, ldb, 0.0f, Cm, ldc,
+                  gemm_pick_tile(M, N, K, false));
+  });
+}
+
 int dmrg_dev_malloc(void** p, size_t bytes) {
   if (!p) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] { DMRG_HIP(hipMalloc(p, bytes)); });

@@ -5,7 +5,7 @@
 //
 //   dmrg2_gpu_amd L chi_max n_sweeps [--josephson --nmax N --ej X --ec X --phi X]
 //                 [--tfim --hfield X] [--quiet]
-//   extras: [--single-site] [--fp32] [--krylov K] [--seed S] [--random] [--neel]
+//   extras: [--single-site] [--fp32 | --bf16x3] [--krylov K] [--seed S] [--random] [--neel]
 //           [--mpo FILE.bin [--mps FILE.bin]]: Hamiltonian (and initial state)
 //           from the reference's benchmark files (benchmarks/lib/data_loader.py:24-88)
 #include <algorithm>
@@ -200,6 +200,7 @@ int main(int argc, char** argv) {
     else if (a == "--quiet") quiet = true;
     else if (a == "--single-site") single = 1;
     else if (a == "--fp32") fp32 = 1;
+    else if (a == "--bf16x3") fp32 = 2;
     else if (a == "--random") init = 1;
     else if (a == "--neel") init = 0;
     else if (a == "--nmax") n_max = atoi(next());
@@ -219,7 +220,7 @@ int main(int argc, char** argv) {
   dmrg_options o;
   dmrg_default_options(&o);
   o.krylov_max = krylov;
-  o.precision = fp32 ? DMRG_PREC_FP32 : DMRG_PREC_FP64;
+  o.precision = fp32 == 2 ? DMRG_PREC_BF16X3 : (fp32 ? DMRG_PREC_FP32 : DMRG_PREC_FP64);
   dmrg_handle h = nullptr;
   int variant = single ? DMRG_SINGLE_SITE : DMRG_TWO_SITE;
   double energy = 0.0;

@@ -1,6 +1,8 @@
 // Contractions of the sweep as MFMA GEMMs (see contract.hpp).
 #include "contract.hpp"
 
+#include <type_traits>
+
 #include <cstring>
 
 namespace dmrg {
@@ -31,15 +33,33 @@ void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, cons
   DMRG_REQUIRE(need <= sc.cap, "heff2: scratch too small");
   const T one(1), zero(0);
   // X[(w + D a'), (ss + dd b)] = sum_a L[a, w, a'] theta[a, ss, b]
-  gemm<T>(st, OP_T, OP_N, D * cL, dd * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
-          pickT<T>(D * cL, dd * cR, cL));
+  if constexpr (std::is_same<T, float>::value) {
+    if (sc.split_bf16) {
+      gemm_f32_split_bf16(st, OP_T, OP_N, D * cL, dd * cR, cL, Lenv, cL, theta, cL, sc.X, D * cL);
+    } else {
+      gemm<T>(st, OP_T, OP_N, D * cL, dd * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
+              pickT<T>(D * cL, dd * cR, cL));
+    }
+  } else {
+    gemm<T>(st, OP_T, OP_N, D * cL, dd * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
+            pickT<T>(D * cL, dd * cR, cL));
+  }
   // Y[a' + cL ss' + cL dd b + cL dd cR n] = sum_{w,ss} X * WW[(w,ss),(ss',n)]
   View2 vx{{cL, (long long)D, (long long)D * cL * dd}, {D, 1, (long long)D * cL}};
   View2 vy{{cL, 1, (long long)cL * dd}, {dd, (long long)cL, (long long)cL * dd * cR}};
   mpo_step<T>(st, cL * cR, dd * D, D * dd, sc.X, vx, WW, sc.Y, vy);
   // out[(a' + cL ss'), b'] = sum_{(b, n)} Y * R[(b + cR n), b']
-  gemm<T>(st, OP_N, OP_N, cL * dd, cR, D * cR, one, sc.Y, cL * dd, Renv, cR * D, zero, out,
-          cL * dd, pickT<T>(cL * dd, cR, D * cR));
+  bool done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (sc.split_bf16) {
+      gemm_f32_split_bf16(st, OP_N, OP_N, cL * dd, cR, D * cR, sc.Y, cL * dd, Renv, cR * D, out,
+                          cL * dd);
+      done = true;
+    }
+  }
+  if (!done)
+    gemm<T>(st, OP_N, OP_N, cL * dd, cR, D * cR, one, sc.Y, cL * dd, Renv, cR * D, zero, out,
+            cL * dd, pickT<T>(cL * dd, cR, D * cR));
 }
 
 template <class T>

@@ -22,6 +22,7 @@ struct ContractScratch {
   T* X = nullptr;
   T* Y = nullptr;
   size_t cap = 0;
+  bool split_bf16 = false;  // FP32 only: the H_eff GEMMs as split-BF16 products
 };
 
 // Two-site H_eff: out = H_eff(theta); WW is the fused two-site MPO as a

@@ -99,7 +99,8 @@ Engine<T>::Engine(int L_, int d_, int chi_, int D_, double tol_, int variant_, c
   DMRG_REQUIRE(chi >= 1 && D >= 1, "bad chi_max / D");
   DMRG_REQUIRE(variant == 1 || variant == 2, "variant must be 1 or 2");
   DMRG_REQUIRE(opt.krylov_max >= 1 && opt.krylov_max <= 128, "krylov_max must be in [1, 128]");
-  DMRG_REQUIRE(opt.precision == 0 || (opt.precision == 1 && !is_cplx<T>::value && variant == 2),
+  DMRG_REQUIRE(opt.precision == 0 ||
+                   ((opt.precision == 1 || opt.precision == 2) && !is_cplx<T>::value && variant == 2),
                "FP32 matvec mode is available for the real two-site engine");
   DMRG_REQUIRE((long long)chi * d <= 2048, "chi_max * d must be <= 2048");
   DMRG_HIP(hipSetDevice(opt.device));
@@ -154,7 +155,7 @@ void Engine<T>::alloc_all() {
     const size_t K = (size_t)D * d * d;
     WWd.assign(L - 1, nullptr);
     for (int i = 0; i < L - 1; ++i) dmalloc((void**)&WWd[i], K * K * sizeof(T));
-    if (opt.precision == 1) {
+    if (opt.precision != 0) {
       WWf.assign(L - 1, nullptr);
       for (int i = 0; i < L - 1; ++i) dmalloc((void**)&WWf[i], K * K * sizeof(float));
       dmalloc((void**)&Lf, env_sz * sizeof(float));
@@ -163,6 +164,7 @@ void Engine<T>::alloc_all() {
       dmalloc((void**)&thf, nmax * sizeof(float));
       dmalloc((void**)&outf, nmax * sizeof(float));
       sc_f.cap = (size_t)D * d * d * chi * chi;
+      sc_f.split_bf16 = opt.precision == 2;
       dmalloc((void**)&sc_f.X, sc_f.cap * sizeof(float));
       dmalloc((void**)&sc_f.Y, sc_f.cap * sizeof(float));
     }
@@ -188,7 +190,7 @@ void Engine<T>::alloc_all() {
   runner.h_scal = h_scal;
   runner.beta_tol = opt.lanczos_tol;
   runner.eig_tol = opt.eig_tol;
-  runner.eig_rel = opt.precision == 1 ? opt.fp32_eig_rel : 0.0;
+  runner.eig_rel = opt.precision != 0 ? opt.fp32_eig_rel : 0.0;
 }
 
 template <class T>
@@ -252,7 +254,7 @@ void Engine<T>::set_mpo(const T* const* W) {
     for (int i = 0; i < L - 1; ++i) {
       build_ww2<T>(W[i], W[i + 1], d, D, ww.data());
       DMRG_HIP(hipMemcpy(WWd[i], ww.data(), K * K * sizeof(T), hipMemcpyHostToDevice));
-      if (opt.precision == 1) {
+      if (opt.precision != 0) {
         for (size_t k = 0; k < K * K; ++k) wwf[k] = (float)real(ww[k]);
         DMRG_HIP(hipMemcpy(WWf[i], wwf.data(), K * K * sizeof(float), hipMemcpyHostToDevice));
       }
@@ -370,7 +372,7 @@ template <class T>
 double Engine<T>::lanczos_two(int site, bool rq) {
   const int cL = bond[site], cR = bond[site + 2];
   const size_t n = (size_t)cL * d * d * cR;
-  if (opt.precision == 1) {
+  if (opt.precision != 0) {
     if constexpr (!is_cplx<T>::value) {
       const size_t nl = (size_t)cL * D * cL, nr = (size_t)cR * D * cR;
       absmax_f64(s_main, Lenv[site], nl, envScale);

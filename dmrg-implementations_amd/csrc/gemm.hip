@@ -11,6 +11,7 @@
 #include "gemm.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace dmrg {
 namespace {
@@ -336,6 +337,163 @@ __global__ __launch_bounds__(256) void k_rows_small(int M, int N, int K, const T
   }
 }
 }  // namespace
+
+namespace {
+// ---------------------------------------------------------------------------
+// Split-BF16 FP32 GEMM: C = A B with every FP32 operand written as
+// hi + mid + lo, three bf16 numbers (round to nearest even, then the exact FP32
+// residual, rounded again), so together they carry 24 significand bits.  The
+// six products hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid (the ones above
+// 2^-24 relative) run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; the
+// result has FP32-class accuracy.  hi*hi goes to its own accumulator.
+//
+// MFMA operand layout (gfx950, checked with random integer inputs): lane l
+// holds row l%16 of the left operand and column l%16 of the right one, over
+// the eight consecutive k = 8*(l/16) .. +7; result register r is row
+// 4*(l/16)+r, column l%16.  B is passed in the left slot, so the result is
+// C^T per lane: lane l -> row l%16 of C, register r -> column 4*(l/16)+r.
+//
+// A(m, k) = A[k + m*lda] (KFA) or A[m + k*lda]; B(k, n) = B[k + n*ldb] (KFB)
+// or B[n + k*ldb]; C column-major.  64x64 workgroup tile, 2x2 waves of 32x32,
+// BK = 32, LDS rows of 40 bf16 (80 B: 16-byte aligned, no bank conflicts for
+// the 16 rows of one 128-bit read).
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+constexpr int BF_BK = 32, BF_LD = 40;
+
+__device__ inline unsigned short bf16_rne(float x) {
+  unsigned u;
+  memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_f(unsigned short h) {
+  const unsigned u = (unsigned)h << 16;
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+__device__ inline void split3(float x, unsigned short& h, unsigned short& m, unsigned short& l) {
+  h = bf16_rne(x);
+  const float r1 = x - bf16_f(h);
+  m = bf16_rne(r1);
+  l = bf16_rne(r1 - bf16_f(m));
+}
+
+template <bool KFA, bool KFB>
+__global__ __launch_bounds__(256) void k_gemm_bf3(int M, int N, int K, const float* __restrict__ A,
+                                                  long long lda, const float* __restrict__ B,
+                                                  long long ldb, float* __restrict__ C,
+                                                  long long ldc) {
+  constexpr int BM = 64, BN = 64, NL = BM * BF_BK / 256;  // 8 elements of A and of B per thread
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][3][BM][BF_LD];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[2][3][BN][BF_LD];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  float ra[NL], rb[NL];
+  auto idx = [&](bool kfast, int r, int& mn, int& k) {
+    if (kfast) { k = t % BF_BK; mn = t / BF_BK + r * (256 / BF_BK); }
+    else { mn = t % 64; k = t / 64 + r * (256 / 64); }
+  };
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      int m, k;
+      idx(KFA, r, m, k);
+      const int gm = m0 + m, gk = k0 + k;
+      ra[r] = (gm < M && gk < K) ? A[KFA ? (gk + (long long)gm * lda) : (gm + (long long)gk * lda)] : 0.f;
+      int n, kb;
+      idx(KFB, r, n, kb);
+      const int gn = n0 + n, gkb = k0 + kb;
+      rb[r] = (gn < N && gkb < K) ? B[KFB ? (gkb + (long long)gn * ldb) : (gn + (long long)gkb * ldb)] : 0.f;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      int m, k;
+      idx(KFA, r, m, k);
+      unsigned short h, mi, lo;
+      split3(ra[r], h, mi, lo);
+      As[buf][0][m][k] = h;
+      As[buf][1][m][k] = mi;
+      As[buf][2][m][k] = lo;
+      int n, kb;
+      idx(KFB, r, n, kb);
+      split3(rb[r], h, mi, lo);
+      Bs[buf][0][n][kb] = h;
+      Bs[buf][1][n][kb] = mi;
+      Bs[buf][2][n][kb] = lo;
+    }
+  };
+  f4_t acc_hh[2][2], acc_x[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc_hh[i][j] = acc_x[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+  const int li = lane & 15, lg = lane >> 4;
+  const int nk = (K + BF_BK - 1) / BF_BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BF_BK);
+    bf8_t af[2][3], bfr[2][3];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        af[i][p] = *reinterpret_cast<const bf8_t*>(&As[buf][p][wm * 32 + i * 16 + li][lg * 8]);
+        bfr[i][p] = *reinterpret_cast<const bf8_t*>(&Bs[buf][p][wn * 32 + i * 16 + li][lg * 8]);
+      }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc_hh[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], af[i][0], acc_hh[i][j], 0, 0, 0);
+        f4_t x = acc_x[i][j];
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], af[i][1], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], af[i][2], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][2], af[i][0], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], af[i][1], x, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], af[i][0], x, 0, 0, 0);
+        acc_x[i][j] = x;
+      }
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm * 32 + i * 16 + li;
+        const int gn = n0 + wn * 32 + j * 16 + lg * 4 + r;
+        if (gm < M && gn < N) C[gm + (long long)gn * ldc] = acc_hh[i][j][r] + acc_x[i][j][r];
+      }
+}
+}  // namespace
+
+void gemm_f32_split_bf16(hipStream_t st, int opA, int opB, int M, int N, int K, const float* A,
+                         int lda, const float* B, int ldb, float* C, int ldc) {
+  DMRG_REQUIRE(M >= 0 && N >= 0 && K >= 0, "gemm_bf16x3: negative size");
+  DMRG_REQUIRE(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "gemm_bf16x3: bad op");
+  if (M == 0 || N == 0) return;
+  // op(A) (m, k): transposed storage is k-fast; op(B) (k, n): plain storage is k-fast
+  const bool kfa = op_trans(opA), kfb = !op_trans(opB);
+  const dim3 grid(cdiv(M, 64), cdiv(N, 64));
+#define DMRG_BF3(KA_, KB_) \
+  hipLaunchKernelGGL((k_gemm_bf3<KA_, KB_>), grid, dim3(256), 0, st, M, N, K, A, (long long)lda, B, \
+                     (long long)ldb, C, (long long)ldc)
+  if (kfa && kfb) DMRG_BF3(true, true);
+  else if (kfa) DMRG_BF3(true, false);
+  else if (kfb) DMRG_BF3(false, true);
+  else DMRG_BF3(false, false);
+#undef DMRG_BF3
+  DMRG_LAUNCH_CHECK();
+}
 
 template <class T>
 bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, const T* B, T* C,

@@ -63,6 +63,13 @@ template <class T>
 bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, const T* B, T* C,
                      View2 vc);
 
+// FP32 C (M x N, ldc) = op(A) op(B) with split-BF16 products: each operand as
+// three bf16 terms, the six leading cross products on the bf16 MFMA with FP32
+// accumulation (FP32-class accuracy).  op: OP_N or OP_T (conjugation is a
+// no-op for real data); alpha = 1, beta = 0.
+void gemm_f32_split_bf16(hipStream_t st, int opA, int opB, int M, int N, int K, const float* A,
+                         int lda, const float* B, int ldb, float* C, int ldc);
+
 // Pick a tile so that the grid covers the chip.
 GemmTile gemm_pick_tile(int M, int N, int K, bool complex_);
 

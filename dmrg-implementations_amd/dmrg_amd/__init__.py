@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(_HERE, "libdmrg_amd.so")
 OK, ERR_HIP, ERR_NO_CONV, ERR_BAD_ARG, ERR_NONFINITE, ERR_STATE, ERR_OOM = range(7)
 DTYPE_F64, DTYPE_C128 = 0, 1
 SINGLE_SITE, TWO_SITE = 1, 2
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 
 EXPORTED = [
     "dmrg_default_options", "dmrg_create", "dmrg_destroy", "dmrg_set_mpo",
@@ -35,7 +35,7 @@ EXPORTED = [
     "dmrg_get_mps", "dmrg_get_env", "dmrg_expectation", "dmrg_get_stats", "dmrg_reset_stats", "dmrg_last_error",
     "dmrg_ctx_create", "dmrg_ctx_destroy", "dmrg_op_heff2", "dmrg_op_heff1", "dmrg_op_env_left",
     "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
-    "dmrg_op_gemm", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h", "dmrg_memcpy_d2d",
+    "dmrg_op_gemm", "dmrg_op_gemm_f32", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h", "dmrg_memcpy_d2d",
     "dmrg_dev_synchronize", "dmrg_device_count",
 ]
 
@@ -114,6 +114,7 @@ def lib():
         "dmrg_op_lanczos2": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp, i, i, C.POINTER(d),
                                  C.POINTER(i)]),
         "dmrg_op_gemm": (i, [vp, vp, i, i, i, i, i, i, vp, vp, i, vp, i, vp, vp, i]),
+        "dmrg_op_gemm_f32": (i, [vp, vp, i, i, i, i, i, i, vp, i, vp, i, vp, i]),
         "dmrg_dev_malloc": (i, [C.POINTER(vp), C.c_size_t]),
         "dmrg_dev_free": (i, [vp]),
         "dmrg_memcpy_h2d": (i, [vp, vp, C.c_size_t]),
@@ -398,6 +399,14 @@ class Ops:
         b = np.array([beta], dtype=npd)
         _check(lib().dmrg_op_gemm(self._h, stream, dt, opA, opB, M, N, K, a.ctypes.data, A.ptr,
                                   A.shape[0], B.ptr, B.shape[0], b.ctypes.data, Cm.ptr, M))
+
+    def gemm_f32(self, opA, opB, A, B, Cm, prec=PREC_FP32, stream=None):
+        """FP32 Cm = op(A) op(B) (op 0 or 1) on float32 device arrays, FP32 MFMA
+        or split-BF16 (prec)."""
+        M, N = Cm.shape
+        K = A.shape[0] if opA == 1 else A.shape[1]
+        _check(lib().dmrg_op_gemm_f32(self._h, stream, prec, opA, opB, M, N, K, A.ptr, A.shape[0],
+                                      B.ptr, B.shape[0], Cm.ptr, M))
 
     def tridiag_lowest(self, alpha, beta):
         k = len(alpha)

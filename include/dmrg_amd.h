@@ -44,8 +44,12 @@ enum {
 enum { DMRG_DTYPE_F64 = 0, DMRG_DTYPE_C128 = 1 };
 enum { DMRG_SINGLE_SITE = 1, DMRG_TWO_SITE = 2 };
 /* Arithmetic of the H_eff matvec.  Environments, Lanczos (tridiagonal solve,
- * reorthogonalisation) and the truncation SVD are FP64 in every mode. */
-enum { DMRG_PREC_FP64 = 0, DMRG_PREC_FP32 = 1 };
+ * reorthogonalisation) and the truncation SVD are FP64 in every mode.
+ * FP32: the matvec GEMMs on FP32 MFMA.  BF16X3: the same FP32 matvec with its
+ * two GEMMs as split-BF16 products (each operand hi+mid+lo in bf16, the six
+ * leading cross products on the bf16 MFMA, FP32 accumulation).  Both mixed
+ * modes are real two-site only. */
+enum { DMRG_PREC_FP64 = 0, DMRG_PREC_FP32 = 1, DMRG_PREC_BF16X3 = 2 };
 
 typedef struct dmrg_options {
   int krylov_max;        /* Lanczos Krylov dimension cap; reference: min(100, n)
@@ -225,6 +229,13 @@ int dmrg_op_lanczos2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR,
 int dmrg_op_gemm(dmrg_ctx_handle c, void* stream, int dtype, int opA, int opB, int M, int N,
                  int K, const void* alpha, const void* A, int lda, const void* B, int ldb,
                  const void* beta, void* Cm, int ldc);
+/* FP32 C = op(A) op(B) (op 0 N, 1 T) as the mixed modes run it: prec
+ * DMRG_PREC_FP32 (FP32 MFMA) or DMRG_PREC_BF16X3 (split-BF16 products).  The
+ * reference has no FP32 GEMM; this is the matvec GEMM of the mixed modes
+ * (their substitute for the rocblas_dgemm calls of apply_heff_two_site,
+ * dmrg2_gpu_impl.h:686-755). */
+int dmrg_op_gemm_f32(dmrg_ctx_handle c, void* stream, int prec, int opA, int opB, int M, int N,
+                     int K, const float* A, int lda, const float* B, int ldb, float* Cm, int ldc);
 
 /* Device-memory helpers so callers need no other HIP binding. */
 int dmrg_dev_malloc(void** p, size_t bytes);

@@ -56,14 +56,16 @@ def test_two_site_matches_oracle_sweep_by_sweep(dmrg_amd):
     eng.close()
 
 
-def test_fp32_matvec_mode_tracks_fp64(dmrg_amd):
-    """FP32 H_eff matvecs with FP64 Lanczos/SVD/environments: the reported
-    energy (FP64 Rayleigh quotient at the last bond) stays within 1e-9 per
-    site of the FP64 run from the same seeded state."""
+@pytest.mark.parametrize("mixed", [1, 2])
+def test_fp32_matvec_mode_tracks_fp64(dmrg_amd, mixed):
+    """FP32 H_eff matvecs (FP32 MFMA, or split-BF16 products) with FP64
+    Lanczos/SVD/environments: the reported energy (FP64 Rayleigh quotient at
+    the last bond) stays within 1e-9 per site of the FP64 run from the same
+    seeded state."""
     L, chi = 16, 32
     Ws = models.heisenberg_mpo(L)
     out = {}
-    for prec in (0, 1):
+    for prec in (0, mixed):
         eng = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20, precision=prec)
         eng.set_mpo(Ws)
         eng.initialize_mps_random(seed=7, scale=0.1)
@@ -72,7 +74,7 @@ def test_fp32_matvec_mode_tracks_fp64(dmrg_amd):
         assert abs(n2 - 1) < 1e-9
         assert abs(e - out[prec]) < 1e-9 * L
         eng.close()
-    assert abs(out[1] - out[0]) < 1e-9 * L, out
+    assert abs(out[mixed] - out[0]) < 1e-9 * L, out
 
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")

@@ -41,6 +41,32 @@ def test_gemm(dmrg_amd, ops, cplx, opA, opB, M, N, K):
     assert rel_err(got, ref) < TOL, (rel_err(got, ref), got.ravel()[:4], ref.ravel()[:4])
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("opA,opB,M,N,K", [(0, 0, 64, 64, 32), (1, 0, 200, 77, 45), (0, 0, 96, 130, 300),
+                                            (1, 1, 33, 65, 129), (0, 1, 2048, 512, 320),
+                                            (1, 0, 2560, 256, 512)])
+def test_gemm_f32_mixed(dmrg_amd, ops, prec, opA, opB, M, N, K):
+    """FP32 GEMM of the mixed matvec: FP32 MFMA (prec 1) and split-BF16
+    products (prec 2, each operand as three bf16 terms, six cross products).
+    Both must reach FP32-class accuracy against the float64 product of the
+    same float32 inputs, |C - C_ref| <= 2e-6 (|A| |B|) elementwise; one bf16
+    term per operand would leave ~4e-3."""
+    rng = np.random.default_rng(M + 7 * N + 13 * K + opA + 2 * opB)
+    A = rng.standard_normal((K, M) if opA else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if opB else (K, N)).astype(np.float32)
+    f = (lambda x: x, lambda x: x.T)
+    A64, B64 = f[opA](A).astype(np.float64), f[opB](B).astype(np.float64)
+    ref = A64 @ B64
+    scale = np.abs(A64) @ np.abs(B64)
+    dA = dmrg_amd.DeviceArray.from_numpy(A, np.float32)
+    dB = dmrg_amd.DeviceArray.from_numpy(B, np.float32)
+    dC = dmrg_amd.DeviceArray((M, N), np.float32)
+    ops.gemm_f32(opA, opB, dA, dB, dC, prec=prec)
+    got = dC.numpy().astype(np.float64)
+    err = np.max(np.abs(got - ref) / scale)
+    assert err < 2e-6, err
+
+
 def _rand_problem(rng, cL, cR, d, D, cplx):
     L = rand(rng, (cL, D, cL), cplx)
     R = rand(rng, (cR, D, cR), cplx)
