@@ -455,7 +455,7 @@ void Engine<T>::warm_alloc() {
   const size_t tsz = (size_t)chi * chi;
   for (T** p : {&wT[0], &wT[1], &wZ0, &wZ1, &wP, &wTmp}) dmalloc((void**)p, tsz * sizeof(T));
   dmalloc((void**)&wXw, capN * capN * sizeof(T));
-  dmalloc((void**)&wDev, kPolarMaxIter * sizeof(double));
+  dmalloc((void**)&wDev, (size_t)kPolarMaxIter * kPolarParts * sizeof(double));
   warm_on = true;
 }
 

@@ -555,15 +555,14 @@ __global__ __launch_bounds__(NT) void k_pair_apply(T* __restrict__ A, T* __restr
   }
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-// max |A(i,j)|, i != j, over the N x N block -> atomicMax on the bit pattern
-// (non-negative doubles order like their bits; NaN maps to +inf bits + 1).
-// With rank given, only pairs with exactly one of rank[i], rank[j] < keep.
+// part[blk] = max |A(i,j)|, i != j, over the workgroup's entries of the N x N
+// block (NaN as +inf); grid of NRED.  With rank given, only pairs with exactly
+// one of rank[i], rank[j] < keep.
 template <class T>
 __global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, int Np,
                                                 const int* __restrict__ rank, int keep,
-                                                unsigned long long* out) {
+                                                double* __restrict__ part) {
   __shared__ double red[256];
   const int t = threadIdx.x;
   double mx = 0.0;
@@ -582,12 +581,7 @@ __global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, 
     if (t < o) red[t] = fmax(red[t], red[t + o]);
     __syncthreads();
   }
-  if (t == 0) {
-    double v = red[0];
-    unsigned long long bits;
-    memcpy(&bits, &v, sizeof(v));
-    atomicMax(out, bits);
-  }
+  if (t == 0) part[blockIdx.x] = red[0];
 }
 
 // out(i, j) = V(i, perm[j]) (conj-transposed into out(j, i) with `herm`).
@@ -749,9 +743,13 @@ __global__ void k_copy_rows(const T* __restrict__ Y, int ldy, int nk, int n, T* 
 // E_ij = A_ij / (lambda_j - lambda_i) for pairs the perturbation resolves
 // (|lambda_j - lambda_i| > sep |A_ij|), 0 otherwise (left to Jacobi).  E is
 // anti-Hermitian, so Q is unitary up to O(|E|^2).
+// part[blk] = sum of |E_ij|^2 over the workgroup's entries (grid of NRED).
 template <class T>
-__global__ void k_oa_q(const T* __restrict__ A, T* __restrict__ Q, int N, int Np, double sep) {
+__global__ __launch_bounds__(256) void k_oa_q(const T* __restrict__ A, T* __restrict__ Q, int N,
+                                              int Np, double sep, double* __restrict__ part) {
+  __shared__ double red[256];
   const size_t tot = (size_t)Np * Np;
+  double e2 = 0.0;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(e % Np), c = (int)(e / Np);
@@ -759,41 +757,45 @@ __global__ void k_oa_q(const T* __restrict__ A, T* __restrict__ Q, int N, int Np
     if (r != c && r < N && c < N) {
       const T a = A[e];
       const double g = real(A[c + (size_t)c * Np]) - real(A[r + (size_t)r * Np]);
-      if (fabs(g) > sep * sqrt(abs2(a))) q = a * (1.0 / g);
+      if (fabs(g) > sep * sqrt(abs2(a))) {
+        q = a * (1.0 / g);
+        e2 += abs2(q);
+      }
     }
     Q[e] = q;
   }
+  red[threadIdx.x] = e2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// dev = max |P - I| (atomic max of the bit pattern); P <- (3I - P)/2.
+// P <- (3I - P)/2 (one Newton-Schulz step's polynomial factor); with part
+// given, part[blk] = sum of |P - I|^2 over the workgroup's entries (grid NRED).
 template <class T>
-__global__ __launch_bounds__(256) void k_ns_step(T* __restrict__ P, int n,
-                                                 unsigned long long* __restrict__ dev) {
+__global__ __launch_bounds__(256) void k_ns_half(T* __restrict__ P, int n, double* __restrict__ part) {
   __shared__ double red[256];
-  const int t = threadIdx.x;
-  double mx = 0.0;
   const size_t tot = (size_t)n * n;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + t; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+  double s2 = 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(e % n), c = (int)(e / n);
     const T one = (r == c) ? T(1) : T(0);
     const T p = P[e];
-    double v = absv(p - one);
-    if (!(v == v)) v = INFINITY;
-    mx = fmax(mx, v);
+    s2 += abs2(p - one);
     P[e] = (3.0 * one - p) * 0.5;
   }
-  red[t] = mx;
+  if (!part) return;
+  red[threadIdx.x] = s2;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) red[t] = fmax(red[t], red[t + o]);
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (t == 0) {
-    double v = red[0];
-    unsigned long long bits;
-    memcpy(&bits, &v, sizeof(v));
-    atomicMax(dev, bits);
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
@@ -846,17 +848,17 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
     return cur;
   }
   double key_hi = 0.0, key_keep = 0.0;  // largest and keep-th largest diagonal entry
+  double* const offpart = w.red + 2 * NRED;  // per-workgroup maxima of k_offmax
+  double* const oapart = w.red + 3 * NRED;   // per-workgroup |E|_F^2 of k_oa_q
   auto offmax = [&](const T* M) {
     if (keep > 0)
       hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, M, N, Np, keep,
                          w.rank, w.red);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w.off);
-    hipLaunchKernelGGL(k_offmax<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0, st,
-                       M, N, Np, keep > 0 ? (const int*)w.rank : nullptr, keep, w.off);
+    hipLaunchKernelGGL(k_offmax<T>, dim3(NRED), dim3(256), 0, st, M, N, Np,
+                       keep > 0 ? (const int*)w.rank : nullptr, keep, offpart);
     DMRG_LAUNCH_CHECK();
-    unsigned long long bits = 0;
-    double kk[2] = {0.0, 0.0};
-    DMRG_HIP(hipMemcpyAsync(&bits, w.off, sizeof(bits), hipMemcpyDeviceToHost, st));
+    double hp[NRED], kk[2] = {0.0, 0.0};
+    DMRG_HIP(hipMemcpyAsync(hp, offpart, sizeof(hp), hipMemcpyDeviceToHost, st));
     if (keep > 0)
       DMRG_HIP(hipMemcpyAsync(kk, w.red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     DMRG_HIP(hipStreamSynchronize(st));
@@ -864,8 +866,8 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
       key_hi = kk[0];
       key_keep = kk[1];
     }
-    double o;
-    std::memcpy(&o, &bits, sizeof(o));
+    double o = 0.0;
+    for (double v : hp) o = std::max(o, v);
     if (std::isinf(o) || !(o == o)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
     return o;
   };
@@ -905,27 +907,39 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
         if (keep > 0 && sep > 30.0) sep = 30.0;
         else break;                                  // Jacobi
       }
-      hipLaunchKernelGGL(k_oa_q<T>, dim3(grid_for(tot)), dim3(256), 0, st, (const T*)w.A[0], w.Y,
-                         N, Np, sep);
-      // make Q unitary: Newton-Schulz Q <- Q (3I - Q^H Q)/2 until max|Q^H Q - I|
-      // was below 1e-8 before the last step (the error then squares to < 1e-16);
-      // the deviations (slots red[2..5]) are read after steps 2 and 4
+      hipLaunchKernelGGL(k_oa_q<T>, dim3(NRED), dim3(256), 0, st, (const T*)w.A[0], w.Y, N, Np, sep,
+                         oapart);
+      // Make Q unitary by Newton-Schulz, Q <- Q (3I - Q^H Q)/2, which maps the
+      // deviation delta = |Q^H Q - I|_2 to (3/4) delta^2 + O(delta^3).  E is
+      // anti-Hermitian, so Q^H Q - I = E^H E and delta_0 <= |E|_F^2: a tiny E
+      // needs one step and no check.  Otherwise the Frobenius norm of
+      // Q^H Q - I (a bound on delta) is read before steps 2 and 4, and the
+      // step after a reading <= 1e-8 is the last.
+      double ep[NRED];
+      DMRG_HIP(hipMemcpyAsync(ep, oapart, sizeof(ep), hipMemcpyDeviceToHost, st));
+      DMRG_HIP(hipStreamSynchronize(st));
+      double e2 = 0.0;
+      for (double v : ep) e2 += v;
+      if (!(e2 == e2)) break;
+      double* const nspart = w.red + 2 * NRED;  // free while no offmax is pending
+      const bool tiny = e2 <= 1e-8;
       bool unitary = false;
-      unsigned long long* nsdev = reinterpret_cast<unsigned long long*>(w.red + 2);
-      for (int ns = 0; ns < 4 && !unitary; ++ns) {
+      for (int ns = 0; ns < (tiny ? 1 : 4) && !unitary; ++ns) {
+        const bool check = !tiny && (ns == 1 || ns == 3);
         gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), w.Y, Np, w.Y, Np, T(0), w.A[1], Np, tl);
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, nsdev + ns);
-        hipLaunchKernelGGL(k_ns_step<T>, dim3(std::min<size_t>(cdiv(tot, 256), 1024)), dim3(256), 0,
-                           st, w.A[1], Np, nsdev + ns);
+        hipLaunchKernelGGL(k_ns_half<T>, dim3(NRED), dim3(256), 0, st, w.A[1], Np,
+                           check ? nspart : nullptr);
         gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, w.A[1], Np, T(0), w.V[1], Np, tl);
         std::swap(w.Y, w.V[1]);
-        if (ns % 2 == 1) {
-          double dv[4];
-          DMRG_HIP(hipMemcpyAsync(dv, w.red + 2, (ns + 1) * sizeof(double), hipMemcpyDeviceToHost,
-                                  st));
+        if (tiny) unitary = true;
+        if (check) {
+          double dp[NRED];
+          DMRG_HIP(hipMemcpyAsync(dp, nspart, sizeof(dp), hipMemcpyDeviceToHost, st));
           DMRG_HIP(hipStreamSynchronize(st));
-          if (!(dv[ns] == dv[ns])) break;
-          unitary = dv[ns] <= 1e-8;
+          double d2 = 0.0;
+          for (double v : dp) d2 += v;
+          if (!(d2 == d2)) break;
+          unitary = d2 <= 1e-16;  // |Q^H Q - I|_F <= 1e-8 before this step
         }
       }
       if (!unitary) break;  // leave it to Jacobi (A, V unchanged)
@@ -1001,11 +1015,10 @@ void SvdWork<T>::alloc(int cap_n) {
   DMRG_HIP(hipMalloc(&perm, cap * sizeof(int)));
   DMRG_HIP(hipMalloc(&rank, cap * sizeof(int)));
   DMRG_HIP(hipMalloc(&gdiag, cap * sizeof(double)));
-  DMRG_HIP(hipMalloc(&red, 2 * NRED * sizeof(double)));
+  DMRG_HIP(hipMalloc(&red, 4 * NRED * sizeof(double)));
   DMRG_HIP(hipMalloc(&sigma, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&resid, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&scal, 4 * sizeof(double)));
-  DMRG_HIP(hipMalloc(&off, sizeof(unsigned long long)));
   DMRG_HIP(hipMalloc(&newk, sizeof(int)));
 }
 
@@ -1016,7 +1029,7 @@ void SvdWork<T>::release() {
     if (V[i]) (void)hipFree(V[i]);
     A[i] = V[i] = nullptr;
   }
-  void* ps[] = {Y, Q, qflag, perm, rank, gdiag, red, sigma, resid, scal, off, newk};
+  void* ps[] = {Y, Q, qflag, perm, rank, gdiag, red, sigma, resid, scal, newk};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   Y = nullptr;
@@ -1029,7 +1042,6 @@ void SvdWork<T>::release() {
   sigma = nullptr;
   resid = nullptr;
   scal = nullptr;
-  off = nullptr;
   newk = nullptr;
   cap = 0;
 }

@@ -46,11 +46,10 @@ struct SvdWork {
   int* perm = nullptr;           // sort permutation
   int* rank = nullptr;           // inverse of perm (position of each index)
   double* gdiag = nullptr;       // squared norms of the k product vectors
-  double* red = nullptr;         // reduction partials / small results (512 doubles)
+  double* red = nullptr;         // reduction partials / small results (1024 doubles)
   double* sigma = nullptr;       // singular values in output order
   double* resid = nullptr;       // first-pass QR residuals
   double* scal = nullptr;        // [0] |rho|_F, [1] discarded weight, [2] trace(rho)
-  unsigned long long* off = nullptr;  // max |rho_ij| (i != j) of a sweep, as bits
   int* newk = nullptr;
   int cap = 0;  // largest supported rho dimension (multiple of 32)
   void alloc(int cap_n);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace dmrg {
 namespace {
@@ -10,12 +11,10 @@ namespace {
 __device__ inline double absv(double a) { return fabs(a); }
 __device__ inline double absv(cdouble a) { return sqrt(a.x * a.x + a.y * a.y); }
 
-__global__ void k_set_one(double* dev) { *dev = 0.0; }  // clears one deviation slot
-
-// dev = max |P - I| (as an atomic max of the bit pattern); P <- (3I - P)/2.
+// part[blk] = max |P - I| over the workgroup's entries (NaN as +inf);
+// P <- (3I - P)/2.  Grid of kPolarParts workgroups.
 template <class T>
-__global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n,
-                                                 unsigned long long* __restrict__ dev) {
+__global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, double* __restrict__ part) {
   __shared__ double red[256];
   const int t = threadIdx.x;
   double mx = 0.0;
@@ -35,12 +34,7 @@ __global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n,
     if (t < o) red[t] = fmax(red[t], red[t + o]);
     __syncthreads();
   }
-  if (t == 0) {
-    double v = red[0];
-    unsigned long long bits;
-    memcpy(&bits, &v, sizeof(v));
-    atomicMax(dev, bits);
-  }
+  if (t == 0) part[blockIdx.x] = red[0];
 }
 
 }  // namespace
@@ -80,24 +74,29 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
   DMRG_HIP(hipMemcpyAsync(Z0, Tin, (size_t)n * n * sizeof(T), hipMemcpyDeviceToDevice, st));
   T* Z = Z0;
   T* Zn = Z1;
-  const unsigned grid = (unsigned)std::min<size_t>(cdiv((size_t)n * n, 256), 1024);
-  // dev[it] = max |Z_it^H Z_it - I|; the host looks at them after the first
-  // step and then every third one (steps past convergence keep Z unitary to
-  // rounding, since the iteration is stable at its fixed point).
-  double hdev[kPolarMaxIter];
+  // dev[it*kPolarParts + blk]: workgroup maxima of |Z_it^H Z_it - I|; the host
+  // reduces them after the first step and then every third one (steps past
+  // convergence keep Z unitary to rounding, the iteration being stable at its
+  // fixed point).
+  static thread_local std::vector<double> hp;
+  hp.resize((size_t)kPolarMaxIter * kPolarParts);
   int seen = 0;
   for (int it = 0; it < kPolarMaxIter; ++it) {
     gemm<T>(st, OP_H, OP_N, n, n, n, T(1), Z, n, Z, n, T(0), P, n, gemm_pick_tile(n, n, n, cx));
-    hipLaunchKernelGGL(k_set_one, dim3(1), dim3(1), 0, st, dev + it);
-    hipLaunchKernelGGL(k_ns_prep<T>, dim3(grid), dim3(256), 0, st, P, n,
-                       reinterpret_cast<unsigned long long*>(dev + it));
+    hipLaunchKernelGGL(k_ns_prep<T>, dim3(kPolarParts), dim3(256), 0, st, P, n,
+                       dev + (size_t)it * kPolarParts);
     DMRG_LAUNCH_CHECK();
     if (it == 0 || it % 3 == 2 || it == kPolarMaxIter - 1) {
-      DMRG_HIP(hipMemcpyAsync(hdev + seen, dev + seen, (it + 1 - seen) * sizeof(double),
+      DMRG_HIP(hipMemcpyAsync(hp.data() + (size_t)seen * kPolarParts, dev + (size_t)seen * kPolarParts,
+                              (size_t)(it + 1 - seen) * kPolarParts * sizeof(double),
                               hipMemcpyDeviceToHost, st));
       DMRG_HIP(hipStreamSynchronize(st));
       for (int j = seen; j <= it; ++j) {
-        const double h = hdev[j];
+        double h = 0.0;
+        for (int b = 0; b < kPolarParts; ++b) {
+          const double v = hp[(size_t)j * kPolarParts + b];
+          h = (v == v) ? std::max(h, v) : INFINITY;
+        }
         // singular values s of Z move as s <- s (3 - s^2)/2: from s << 1 about
         // log_1.5(1/s) steps, then quadratically; a zero singular value never
         // converges, so give up when the start is too far or progress stalls.

@@ -32,13 +32,15 @@ void overlap_right_step(hipStream_t st, const T* Tin, int r, int c, const T* Bol
                         const T* Bnew, int c2, int d, T* Tout, T* tmp);
 
 constexpr int kPolarMaxIter = 40;
+constexpr int kPolarParts = 256;  // workgroups of the deviation reduction
 
 // Unitary polar factor of the n x n matrix Tin by Newton-Schulz iteration
 // Z <- Z (3I - Z^H Z)/2.  Returns false (and leaves *Zout undefined) when
 // max|T^H T - I| > start_tol, when the iteration stalls (a singular value
 // near 0) or when it does not reach max|Z^H Z - I| <= 1e-14 in kPolarMaxIter
-// steps.  Z0, Z1, P are n x n work buffers; dev holds kPolarMaxIter device
-// doubles.  Synchronises the stream after the first step and every third.
+// steps.  Z0, Z1, P are n x n work buffers; dev holds kPolarMaxIter *
+// kPolarParts device doubles.  Synchronises the stream after the first step
+// and every third.
 template <class T>
 bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
                    double start_tol, T** Zout);
