@@ -141,6 +141,7 @@ def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
     # one sweep per call: each call rebuilds the environments (outside the
     # sweep timer, as in the reference) and times exactly one full sweep
     per_sweep = []
+    E = eng.get_energy()
     t0 = time.perf_counter()
     for _ in range(steps):
         E = eng.run(1)
