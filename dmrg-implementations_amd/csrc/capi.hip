@@ -56,6 +56,7 @@ Options to_opts(const dmrg_options* o) {
     r.svd_warm = o->svd_warm;
     r.fp32_eig_rel = o->fp32_eig_rel;
     r.svd_subspace = o->svd_subspace;
+    r.svd_tol_mixed = o->svd_tol_mixed;
   }
   return r;
 }
@@ -91,6 +92,7 @@ void dmrg_default_options(dmrg_options* o) {
   o->svd_warm = d.svd_warm;
   o->fp32_eig_rel = d.fp32_eig_rel;
   o->svd_subspace = d.svd_subspace;
+  o->svd_tol_mixed = d.svd_tol_mixed;
 }
 
 const char* dmrg_last_error(void) { return g_err.c_str(); }

@@ -32,6 +32,8 @@ struct Options {
   // Truncation: converge only the kept/discarded coupling of each split
   // (same truncated state; the kept vectors are a basis of the kept space).
   int svd_subspace = 1;
+  // Jacobi stop of the split in the FP32 / BF16X3 matvec modes.
+  double svd_tol_mixed = 1e-13;
 };
 
 struct Stats {
@@ -149,7 +151,7 @@ class Engine {
   SvdParams svd_params() const {
     SvdParams p;
     p.max_sweeps = opt.svd_max_sweeps;
-    p.tol = opt.svd_tol;
+    p.tol = opt.precision != 0 ? opt.svd_tol_mixed : opt.svd_tol;
     p.subspace = opt.svd_subspace != 0;
     return p;
   }

@@ -52,7 +52,7 @@ class Options(C.Structure):
                 ("precision", C.c_int), ("profile", C.c_int), ("device", C.c_int),
                 ("svd_max_sweeps", C.c_int), ("svd_tol", C.c_double),
                 ("svd_warm", C.c_int), ("fp32_eig_rel", C.c_double),
-                ("svd_subspace", C.c_int)]
+                ("svd_subspace", C.c_int), ("svd_tol_mixed", C.c_double)]
 
 
 class Stats(C.Structure):

@@ -73,6 +73,11 @@ typedef struct dmrg_options {
                             truncated state is the same, the kept tensor is an
                             orthonormal basis of the kept space rather than the
                             singular vectors.  0: converge every pair.       */
+  double svd_tol_mixed;  /* Jacobi stop in the FP32 / BF16X3 modes (1e-13): the
+                            two-site vectors there carry FP32 rounding (~1e-7),
+                            which couples kept and discarded directions near
+                            the cut at ~1e-13 |rho|_F; converging further would
+                            resolve that rounding.                            */
 } dmrg_options;
 
 typedef struct dmrg_stats {
