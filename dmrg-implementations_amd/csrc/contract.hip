@@ -57,6 +57,10 @@ void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, cons
       done = true;
     }
   }
+  // X is free again here: it serves as the split-K workspace
+  if (!done)
+    done = gemm_nn_splitk<T>(st, cL * dd, cR, D * cR, sc.Y, cL * dd, Renv, cR * D, out, cL * dd,
+                             sc.X, sc.cap);
   if (!done)
     gemm<T>(st, OP_N, OP_N, cL * dd, cR, D * cR, one, sc.Y, cL * dd, Renv, cR * D, zero, out,
             cL * dd, pickT<T>(cL * dd, cR, D * cR));

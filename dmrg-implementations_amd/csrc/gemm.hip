@@ -495,6 +495,44 @@ void gemm_f32_split_bf16(hipStream_t st, int opA, int opB, int M, int N, int K, 
   DMRG_LAUNCH_CHECK();
 }
 
+namespace {
+// C[i] = sum_{z<S} W[i + z*stride] for i < M*N over an M x N block (ld ldc);
+// fixed summation order (deterministic).
+template <class T>
+__global__ void k_splitk_sum(const T* __restrict__ W, int S, long long stride, int M, int N,
+                             T* __restrict__ C, int ldc) {
+  const long long tot = (long long)M * N;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    T v = W[e];
+    for (int z = 1; z < S; ++z) v = v + W[e + z * stride];
+    C[(e % M) + (e / M) * (long long)ldc] = v;
+  }
+}
+}  // namespace
+
+template <class T>
+bool gemm_nn_splitk(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
+                    T* C, int ldc, T* ws, size_t ws_elems) {
+  // S equal slices (S | K) in one batched launch, while the 64x64 grid
+  // leaves CUs idle and each slice keeps K/S >= 512
+  const long long t64 = (long long)cdiv(M, 64) * cdiv(N, 64);
+  int S = 1;
+  while (S < 8 && t64 * S * 2 <= 1024 && K % (S * 2) == 0 && K / (S * 2) >= 512 &&
+         (size_t)(S * 2) * M * N <= ws_elems)
+    S *= 2;
+  if (S == 1) return false;
+  const int Kc = K / S;
+  // slice z: A(:, z Kc ..) (A + z Kc lda), B(z Kc .., :) (B + z Kc) -> ws + z M N
+  gemm_strided<T>(st, OP_N, OP_N, M, N, Kc, T(1), A, lda, (long long)Kc * lda, B, ldb,
+                  (long long)Kc, T(0), ws, M, (long long)M * N, S, GemmTile{64, 64});
+  const long long tot = (long long)M * N;
+  hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)std::min<long long>(cdiv(tot, 256), 4096)),
+                     dim3(256), 0, st, (const T*)ws, S, (long long)M * N, M, N, C, ldc);
+  DMRG_LAUNCH_CHECK();
+  return true;
+}
+
 template <class T>
 bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, const T* B, T* C,
                      View2 vc) {
@@ -574,7 +612,9 @@ void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2
   template void gemm_tensor<T>(hipStream_t, int, int, int, T, const T*, View2, bool, bool,     \
                                const T*, View2, bool, bool, T, T*, View2, GemmTile);           \
   template bool gemm_rows_small<T>(hipStream_t, int, int, int, const T*, View2, const T*, T*,  \
-                                   View2);
+                                   View2);                                                      \
+  template bool gemm_nn_splitk<T>(hipStream_t, int, int, int, const T*, int, const T*, int, T*, \
+                                  int, T*, size_t);
 DMRG_INST(double)
 DMRG_INST(cdouble)
 DMRG_INST(float)

@@ -70,6 +70,14 @@ bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, 
 void gemm_f32_split_bf16(hipStream_t st, int opA, int opB, int M, int N, int K, const float* A,
                          int lda, const float* B, int ldb, float* C, int ldc);
 
+// C = A B (both op N, alpha 1, beta 0) split along K into slices computed
+// into the workspace ws (ws_elems elements) and summed in a fixed order:
+// for K-heavy products whose 64x64 grid cannot fill the chip.  Returns false
+// (and does nothing) when splitting would not help or ws is too small.
+template <class T>
+bool gemm_nn_splitk(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
+                    T* C, int ldc, T* ws, size_t ws_elems);
+
 // Pick a tile so that the grid covers the chip.
 GemmTile gemm_pick_tile(int M, int N, int K, bool complex_);
 
