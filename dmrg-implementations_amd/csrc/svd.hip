@@ -58,9 +58,6 @@ inline int sched_rounds(int nb, int ka) {
 }
 // Global row/column of local index i (0..JP-1) of the pair (I, J).
 __device__ inline int gidx(int I, int J, int i) { return i < JB ? I * JB + i : J * JB + (i - JB); }
-// Inner circle schedule over the JP local indices: round rd pairs positions
-// (q, JP-1 - q); position 0 is fixed and the others rotate.
-__device__ inline int cpos(int pos, int rd) { return pos == 0 ? 0 : 1 + (pos - 1 + rd) % (JP - 1); }
 
 __device__ inline double absv(double a) { return fabs(a); }
 __device__ inline double absv(cdouble a) { return sqrt(a.x * a.x + a.y * a.y); }
@@ -239,14 +236,79 @@ __global__ void k_permute(const T* __restrict__ A, const T* __restrict__ V, T* _
 // c = 1/sqrt(1+t^2), s = t c, J = [[c, s e^{i phi}], [-s e^{-i phi}, c]];
 // J^H [[a_aa, beta], [conj(beta), a_bb]] J = diag(a_aa - t|beta|, a_bb + t|beta|)
 // (the symmetric Schur decomposition with a phase, Golub & Van Loan 8.4).
+// Evaluated as u = |a_bb - a_aa|, r = sqrt(u^2 + 4|beta|^2),
+// y = 1/sqrt(2 r (u + r)): c = (u + r) y, s e^{i phi} = 2 sign(tau) beta y,
+// t|beta| = 4 sign(tau) |beta|^2 r y^2 (t = 2 sign(tau) |beta| / (u + r) and
+// 1/(u + r) = 2 r y^2).  Every term is a sum of non-negative numbers, so there
+// is no cancellation, also for |beta| << u, and c^2 + |s|^2 = 1 to the
+// accuracy of y.
 //
+// Inner schedule: the circle method over the JP local indices.  Round g pairs
+// positions (p, JP-1-p), p < HP = JP/2; the index at each position advances
+// by adv() from round to round (position == index at round 0), so the pair p
+// of round g+1 joins the indices at positions adv(p) and adv(JP-1-p) of
+// round g, which lie in two different pairs of round g.
 //
-// NTA threads (4 waves) per sub-problem.  Each inner round: lanes < 16 compute
-// the round's 16 rotations (identity when |a_ab| <= thr), barrier, every
-// thread updates its 2x2 blocks and Q entries of rotated pairs, barrier.  A
-// sub-problem whose off-diagonal is already below 1e-6 |rho|_F gets one inner
-// sweep (the next outer sweep re-checks it), others up to inner_max.
+// NTA threads (4 waves) per sub-problem, one barrier per round.  Threads
+// t < NBLK each own one unordered pair of pairs {k1 < k2} of the round and
+// replace the 2x2 block rho(k1 rows, k2 cols) by J1^H X J2 (and the mirror
+// block by its conjugate transpose, which keeps the sub-matrix exactly
+// Hermitian).  The element that pairs the next round's pair p is in exactly
+// one such block (a fixed block for each p, whatever the round): its owner
+// computes the rotation of round g+1 from the updated value still in
+// registers, writes it into the other half of the double-buffered (c, s)
+// table, zeroes the element and its mirror and moves the two diagonal entries.
+// No other thread touches those entries in that round.  Threads QT0 and up
+// (waves 2-3) apply the round's rotations to the columns of Q.  A pair with |a_ab| <= thr gets (c, s) = (1, 0), which leaves every
+// entry bit-for-bit unchanged, so the updates need no flags.  A sub-problem
+// whose off-diagonal is already below 1e-6 |rho|_F gets one inner sweep (the
+// next outer sweep re-checks it), others up to inner_max; a sweep without a
+// rotation ends the loop.
 constexpr int NTA = 256;
+constexpr int HP = JP / 2;                  // pairs per inner round
+constexpr int NBLK = HP * (HP - 1) / 2;     // off-diagonal 2x2 blocks per round
+constexpr int QT0 = 128;                    // first thread of the Q updates
+constexpr int NQP = JP * HP / (NTA - QT0);  // Q column-pair updates per Q thread
+static_assert(NBLK <= QT0 && (JP * HP) % (NTA - QT0) == 0 && (NTA - QT0) % JP == 0,
+              "thread mapping of the pair kernel");
+
+__device__ inline int adv_pos(int i) { return i == 0 ? 0 : (i == JP - 1 ? 1 : i + 1); }
+__device__ inline int pair_of(int pos) { return pos < HP ? pos : JP - 1 - pos; }
+
+// 1/sqrt(x) for positive normal x: the hardware estimate refined by one
+// third-order step, e = 1 - x y^2, y <- y (1 + e/2 + 3 e^2/8).
+__device__ inline double rsqrt_refined(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// Rotation of one pair (formulas above); false (and the identity) when
+// |beta|^2 <= thr2.
+template <class T>
+__device__ inline bool pair_rotation(double al, double ga, T be, double thr2, double& c, T& sp,
+                                     double& tb) {
+  const double m2 = abs2(be);
+  c = 1.0;
+  sp = T(0);
+  tb = 0.0;
+  if (!(m2 > thr2)) return false;
+  const double dl = ga - al, u = fabs(dl), sg = dl >= 0.0 ? 1.0 : -1.0;
+  const double r2 = fma(dl, dl, 4.0 * m2);
+  if (r2 > 0x1p-900 && r2 < 0x1p+900) {
+    const double r = r2 * rsqrt_refined(r2);
+    const double y = rsqrt_refined(2.0 * r * (u + r));
+    c = (u + r) * y;
+    sp = be * (2.0 * sg * y);
+    tb = 4.0 * sg * m2 * r * y * y;
+  } else {  // far out of the usual range: library square root and division
+    const double w = 1.0 / (u + sqrt(r2));
+    tb = 2.0 * sg * m2 * w;
+    c = 1.0 / sqrt(1.0 + 4.0 * m2 * w * w);
+    sp = be * (2.0 * sg * c * w);
+  }
+  return true;
+}
 
 template <class T>
 __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int nb, int ka,
@@ -256,10 +318,9 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
                                                   int inner_max) {
   __shared__ T As[JP * LDP];  // As[c * LDP + r] = sub(r, c)
   __shared__ T Qs[JP * LDP];
-  __shared__ double rc[JP / 2];
-  __shared__ T rs[JP / 2];
-  __shared__ int rf[JP / 2];
-  __shared__ int s_rot;
+  __shared__ double rc[2][HP];
+  __shared__ T rs[2][HP];
+  __shared__ int s_rot[2];  // a rotation happened in an even / odd inner sweep
   const int k = blockIdx.x, t = threadIdx.x;
   if (pair_is_idle(nb, ka, k)) {
     if (t == 0) qflag[k] = 0;
@@ -283,123 +344,175 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
       far |= a2v > near2;
     }
   }
-  if (t == 0) s_rot = 0;
+  if (t < 2) s_rot[t] = 0;
   if (!__syncthreads_or(big)) {
     if (t == 0) qflag[k] = 0;
     return;
   }
   const int sweeps = __syncthreads_or(far) ? inner_max : 1;
-  // Circle-schedule indices, advanced incrementally each round (index 0 is
-  // fixed; the others step 1 -> 2 -> ... -> JP-1 -> 1), starting from
-  // position == index.  After JP-1 rounds they are back at the start.
-  auto adv = [](int i) { return i == 0 ? 0 : (i == JP - 1 ? 1 : i + 1); };
-  // HP = JP/2 pairs per round.  Thread t updates the 2x2 blocks of row pair
-  // k1 = t / TPR and column pairs k2 = t % TPR + TPR*q (TPR = NTA/HP threads
-  // per row pair), and the Q entries of row t % JP in pairs
-  // kk = t / JP + (NTA/JP)*q.
-  constexpr int HP = JP / 2;
-  constexpr int TPR = NTA / HP;
-  constexpr int NBQ = HP / TPR, NQQ = JP * HP / NTA;
-  static_assert(NTA % HP == 0 && HP % TPR == 0 && NTA % JP == 0 && (JP * HP) % NTA == 0,
-                "mapping");
-  const int k1 = t / TPR;
-  int a1 = k1, b1 = JP - 1 - k1;
-  int a2[NBQ], b2[NBQ], aq[NQQ], bq[NQQ];
-#pragma unroll
-  for (int q = 0; q < NBQ; ++q) { a2[q] = t % TPR + TPR * q; b2[q] = JP - 1 - a2[q]; }
-#pragma unroll
-  for (int q = 0; q < NQQ; ++q) { aq[q] = t / JP + (NTA / JP) * q; bq[q] = JP - 1 - aq[q]; }
-  int ap = t % HP, bp = JP - 1 - (t % HP);
-  const int row = t % JP;
-  int any = 0;
-  for (int sw = 0; sw < sweeps; ++sw) {
-    for (int rd = 0; rd < JP - 1; ++rd) {
-      if (t < HP) {
-        const int a = ap, b = bp;
-        const double al = real(As[a * LDP + a]), ga = real(As[b * LDP + b]);
-        const T be = As[b * LDP + a];
-        const double mag2 = abs2(be);
-        double c = 1.0;
-        T sp = T(0);
-        int f = 0;
-        if (mag2 > thr2) {
-          const double dl = ga - al;
-          const double sg = dl >= 0.0 ? 1.0 : -1.0;
-          double tb;
-          if (mag2 < 1e-18 * dl * dl) {
-            // |beta| << |dl|: t = beta/dl to O(t^3), c = 1 - t^2/2 to O(t^4),
-            // both exact to rounding here (t^2 < 1e-18)
-            const double inv = 1.0 / dl;
-            tb = mag2 * inv;
-            c = 1.0 - 0.5 * mag2 * inv * inv;
-            sp = be * (c * inv);
-          } else {
-            // t = sign(tau)/(|tau| + sqrt(1+tau^2)) with tau = dl/(2|beta|), written
-            // as t = 2|beta| sgn w, w = 1/(|dl| + sqrt(dl^2 + 4|beta|^2)), so one
-            // sqrt, one reciprocal and one rsqrt give c, s e^{i phi} and t|beta|.
-            const double w = 1.0 / (fabs(dl) + sqrt(dl * dl + 4.0 * mag2));
-            tb = 2.0 * sg * mag2 * w;                // t |beta|
-            c = rsqrt(1.0 + 4.0 * mag2 * w * w);     // 1/sqrt(1+t^2)
-            sp = be * (2.0 * sg * c * w);            // s e^{i phi} = t c beta/|beta|
-          }
-          As[a * LDP + a] = T(al - tb);
-          As[b * LDP + b] = T(ga + tb);
-          As[b * LDP + a] = T(0);
-          As[a * LDP + b] = T(0);
-          f = 1;
-          s_rot = 1;
-        }
-        rc[t] = c;
-        rs[t] = sp;
-        rf[t] = f;
+  constexpr int RPS = JP - 1;  // rounds per inner sweep
+  const int total = sweeps * RPS;
+
+  // Block threads: rows from pair kr, columns from pair kc (k1 < k2 of the
+  // thread's pair of pairs).  The owner of the next-round pair `own` orients
+  // its block so that the element of that pair is the (0, 0) entry: rows
+  // (r0, r1) with r0 the index of the pair's first position, columns
+  // (c0, c1) with c0 the second; a pair taken in the order (b, a) has the
+  // rotation [[c, -conj(s)], [s, c]] (fr, fc).  r0.. c1 are the indices at
+  // those positions, advanced with the schedule.
+  int kr = 0, kc = 0, own = -1;
+  bool fr = false, fc = false;
+  if (t < NBLK) {
+    int k1 = 0, rem = t;
+    while (rem >= HP - 1 - k1) { rem -= HP - 1 - k1; ++k1; }
+    kr = k1;
+    kc = k1 + 1 + rem;
+    const int k2 = kc;
+    for (int p = 0; p < HP; ++p) {
+      const int u = adv_pos(p), v = adv_pos(JP - 1 - p);
+      const int pu = pair_of(u), pv = pair_of(v);
+      if ((pu == k1 && pv == k2) || (pu == k2 && pv == k1)) {
+        own = p;
+        kr = pu;
+        kc = pv;
+        fr = u != pu;
+        fc = v != pv;
       }
-      __syncthreads();
-      // Off-diagonal 2x2 blocks: X <- J1^H X J2.
-      const int f1 = rf[k1];
-      const double c1 = rc[k1];
-      const T s1 = rs[k1];
-#pragma unroll
-      for (int q = 0; q < NBQ; ++q) {
-        const int k2 = t % TPR + TPR * q;
-        if (k1 == k2 || !(f1 | rf[k2])) continue;
-        const T x00 = As[a2[q] * LDP + a1], x01 = As[b2[q] * LDP + a1];
-        const T x10 = As[a2[q] * LDP + b1], x11 = As[b2[q] * LDP + b1];
-        const double c2 = rc[k2];
-        const T s2 = rs[k2];
-        const T y00 = c2 * x00 - conj(s2) * x01, y01 = s2 * x00 + c2 * x01;
-        const T y10 = c2 * x10 - conj(s2) * x11, y11 = s2 * x10 + c2 * x11;
-        As[a2[q] * LDP + a1] = c1 * y00 - s1 * y10;
-        As[b2[q] * LDP + a1] = c1 * y01 - s1 * y11;
-        As[a2[q] * LDP + b1] = conj(s1) * y00 + c1 * y10;
-        As[b2[q] * LDP + b1] = conj(s1) * y01 + c1 * y11;
-      }
-      // Q <- Q J.
-#pragma unroll
-      for (int q = 0; q < NQQ; ++q) {
-        const int kk = t / JP + (NTA / JP) * q;
-        if (!rf[kk]) continue;
-        const T qa = Qs[aq[q] * LDP + row], qb = Qs[bq[q] * LDP + row];
-        const double c = rc[kk];
-        const T s = rs[kk];
-        Qs[aq[q] * LDP + row] = c * qa - conj(s) * qb;
-        Qs[bq[q] * LDP + row] = s * qa + c * qb;
-      }
-      a1 = adv(a1);
-      b1 = adv(b1);
-#pragma unroll
-      for (int q = 0; q < NBQ; ++q) { a2[q] = adv(a2[q]); b2[q] = adv(b2[q]); }
-#pragma unroll
-      for (int q = 0; q < NQQ; ++q) { aq[q] = adv(aq[q]); bq[q] = adv(bq[q]); }
-      ap = adv(ap);
-      bp = adv(bp);
-      __syncthreads();
     }
-    const int rot = s_rot;
+  }
+  int r0 = fr ? JP - 1 - kr : kr, r1 = fr ? kr : JP - 1 - kr;
+  int c0 = fc ? JP - 1 - kc : kc, c1 = fc ? kc : JP - 1 - kc;
+  // Q threads: row qrow of Q, column pairs kq (indices aq[q], bq[q]).
+  const int tq = t - QT0;
+  const int qrow = tq >= 0 ? tq % JP : 0;
+  int aq[NQP], bq[NQP];
+#pragma unroll
+  for (int q = 0; q < NQP; ++q) {
+    const int kk = (tq >= 0 ? tq / JP : 0) + ((NTA - QT0) / JP) * q;
+    aq[q] = kk;
+    bq[q] = JP - 1 - kk;
+  }
+
+  // Rotations of round 0.
+  if (t < HP) {
+    const int a = t, b = JP - 1 - t;
+    const double al = real(As[a * LDP + a]), ga = real(As[b * LDP + b]);
+    double c, tb;
+    T sp;
+    if (pair_rotation(al, ga, As[b * LDP + a], thr2, c, sp, tb)) {
+      As[a * LDP + a] = T(al - tb);
+      As[b * LDP + b] = T(ga + tb);
+      As[b * LDP + a] = T(0);
+      As[a * LDP + b] = T(0);
+      s_rot[0] = 1;
+    }
+    rc[0][t] = c;
+    rs[0][t] = sp;
+  }
+  __syncthreads();
+
+  int any = 0, rin = 0, sw = 0;  // round within the sweep, sweep
+  for (int g = 0; g < total; ++g) {
+    const int cb = g & 1;
+    if (t < NBLK) {
+      // every load of the round first
+      const double cr = rc[cb][kr], cc = rc[cb][kc];
+      T sr = rs[cb][kr], sc = rs[cb][kc];
+      const T x00 = As[c0 * LDP + r0], x01 = As[c1 * LDP + r0];
+      const T x10 = As[c0 * LDP + r1], x11 = As[c1 * LDP + r1];
+      const double al = real(As[r0 * LDP + r0]), ga = real(As[c0 * LDP + c0]);
+      if (fr) sr = -conj(sr);
+      if (fc) sc = -conj(sc);
+      // O = Jr^H X Jc
+      const T y00 = cc * x00 - conj(sc) * x01, y01 = sc * x00 + cc * x01;
+      const T y10 = cc * x10 - conj(sc) * x11, y11 = sc * x10 + cc * x11;
+      T o00 = cr * y00 - sr * y10;
+      const T o01 = cr * y01 - sr * y11;
+      const T o10 = conj(sr) * y00 + cr * y10, o11 = conj(sr) * y01 + cr * y11;
+      // Rotation of the next round from o00, evaluated by every block lane
+      // on safe stand-in values where it is not needed (no divergence); the
+      // owner keeps it.
+      const bool live = own >= 0 && g + 1 < total;
+      const double m2 = abs2(o00);
+      const bool rot = live && m2 > thr2;
+      double c = 1.0, tb = 0.0;
+      T sp = T(0);
+      {
+        const double dl = rot ? ga - al : 0.0, m2s = rot ? m2 : 1.0;
+        const double u = fabs(dl), sg = dl >= 0.0 ? 1.0 : -1.0;
+        const double r2 = fma(dl, dl, 4.0 * m2s);
+        double cw, yw, tw;
+        if (r2 > 0x1p-900 && r2 < 0x1p+900) {
+          const double r = r2 * rsqrt_refined(r2);
+          const double y = rsqrt_refined(2.0 * r * (u + r));
+          cw = (u + r) * y;
+          yw = 2.0 * sg * y;
+          tw = 4.0 * sg * m2s * r * y * y;
+        } else {  // far out of the usual range: library square root and division
+          const double w = 1.0 / (u + sqrt(r2));
+          tw = 2.0 * sg * m2s * w;
+          cw = 1.0 / sqrt(1.0 + 4.0 * m2s * w * w);
+          yw = 2.0 * sg * cw * w;
+        }
+        if (rot) {
+          c = cw;
+          sp = o00 * yw;
+          tb = tw;
+          o00 = T(0);
+        }
+      }
+      if (live) {
+        As[r0 * LDP + r0] = T(al - tb);
+        As[c0 * LDP + c0] = T(ga + tb);
+        rc[cb ^ 1][own] = c;
+        rs[cb ^ 1][own] = sp;
+        if (rot) s_rot[(sw + (rin + 1 == RPS)) & 1] = 1;
+      }
+      As[c0 * LDP + r0] = o00;
+      As[c1 * LDP + r0] = o01;
+      As[c0 * LDP + r1] = o10;
+      As[c1 * LDP + r1] = o11;
+      As[r0 * LDP + c0] = conj(o00);
+      As[r0 * LDP + c1] = conj(o01);
+      As[r1 * LDP + c0] = conj(o10);
+      As[r1 * LDP + c1] = conj(o11);
+    } else if (tq >= 0) {
+      // Q <- Q J (all loads first: the stores go to other entries)
+      T qa[NQP], qb[NQP], sq[NQP];
+      double cq[NQP];
+#pragma unroll
+      for (int q = 0; q < NQP; ++q) {
+        const int kk = tq / JP + ((NTA - QT0) / JP) * q;
+        cq[q] = rc[cb][kk];
+        sq[q] = rs[cb][kk];
+        qa[q] = Qs[aq[q] * LDP + qrow];
+        qb[q] = Qs[bq[q] * LDP + qrow];
+      }
+#pragma unroll
+      for (int q = 0; q < NQP; ++q) {
+        Qs[aq[q] * LDP + qrow] = cq[q] * qa[q] - conj(sq[q]) * qb[q];
+        Qs[bq[q] * LDP + qrow] = sq[q] * qa[q] + cq[q] * qb[q];
+      }
+    }
+    // The flag of sweep sw-1 is free again once every thread has read it at
+    // the end of that sweep (one barrier ago at least).
+    if (t == 0 && rin == 1 && sw > 0) s_rot[(sw - 1) & 1] = 0;
+    r0 = adv_pos(r0);
+    r1 = adv_pos(r1);
+    c0 = adv_pos(c0);
+    c1 = adv_pos(c1);
+#pragma unroll
+    for (int q = 0; q < NQP; ++q) { aq[q] = adv_pos(aq[q]); bq[q] = adv_pos(bq[q]); }
     __syncthreads();
-    if (t == 0) s_rot = 0;
-    __syncthreads();
-    any |= rot;
-    if (!rot) break;
+    if (++rin == RPS) {
+      // sweep sw ends: its flag, and the next sweep's (set only by the
+      // rotations of that sweep's first round, already applied)
+      const int rot = s_rot[sw & 1] | s_rot[(sw + 1) & 1];
+      any |= rot;
+      rin = 0;
+      ++sw;
+      if (!rot) break;
+    }
   }
   if (t == 0) qflag[k] = any;
   if (!any) return;
@@ -409,7 +522,6 @@ __global__ __launch_bounds__(NTA) void k_pair_eig(T* __restrict__ A, int ld, int
     Qout[(size_t)k * JP * JP + e] = Qs[c * LDP + r];
   }
 }
-
 // ---------------------------------------------------------------------------
 // Kernel B: apply the round's block-diagonal unitary P (pair blocks Q_P):
 //   off-diagonal pair blocks  rho_PQ <- Q_P^H rho_PQ Q_Q  (and the mirror),
