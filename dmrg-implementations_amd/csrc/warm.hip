@@ -12,9 +12,10 @@ __device__ inline double absv(double a) { return fabs(a); }
 __device__ inline double absv(cdouble a) { return sqrt(a.x * a.x + a.y * a.y); }
 
 // part[blk] = max |P - I| over the workgroup's entries (NaN as +inf);
-// P <- (3I - P)/2.  Grid of kPolarParts workgroups.
+// P <- (a/2)(3I - a^2 P).  Grid of kPolarParts workgroups.
 template <class T>
-__global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, double a,
+                                                 double* __restrict__ part) {
   __shared__ double red[256];
   const int t = threadIdx.x;
   double mx = 0.0;
@@ -26,7 +27,7 @@ __global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, doubl
     double v = absv(p - one);
     if (!(v == v)) v = INFINITY;
     mx = fmax(mx, v);
-    P[e] = (3.0 * one - p) * 0.5;
+    P[e] = (3.0 * one - (a * a) * p) * (0.5 * a);
   }
   red[t] = mx;
   __syncthreads();
@@ -75,18 +76,22 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
   T* Z = Z0;
   T* Zn = Z1;
   // dev[it*kPolarParts + blk]: workgroup maxima of |Z_it^H Z_it - I|; the host
-  // reduces them after the first step and then every third one (steps past
-  // convergence keep Z unitary to rounding, the iteration being stable at its
-  // fixed point).
+  // reduces them after the first step and after steps 9, 11, 13, ... (from
+  // l = 1e-3 the scaled iteration needs about 12; steps past convergence keep
+  // Z unitary to rounding, the iteration being stable at its fixed point, and
+  // by then a = 1: the bound l reaches 1 no later than the singular values).
   static thread_local std::vector<double> hp;
   hp.resize((size_t)kPolarMaxIter * kPolarParts);
   int seen = 0;
+  double l = kPolarL0;  // lower bound of the smallest singular value of Z
   for (int it = 0; it < kPolarMaxIter; ++it) {
+    const double a = l < 1.0 ? std::sqrt(3.0 / (1.0 + l + l * l)) : 1.0;
+    l = std::min(1.0, 0.5 * a * l * (3.0 - a * a * l * l));
     gemm<T>(st, OP_H, OP_N, n, n, n, T(1), Z, n, Z, n, T(0), P, n, gemm_pick_tile(n, n, n, cx));
-    hipLaunchKernelGGL(k_ns_prep<T>, dim3(kPolarParts), dim3(256), 0, st, P, n,
+    hipLaunchKernelGGL(k_ns_prep<T>, dim3(kPolarParts), dim3(256), 0, st, P, n, a,
                        dev + (size_t)it * kPolarParts);
     DMRG_LAUNCH_CHECK();
-    if (it == 0 || it % 3 == 2 || it == kPolarMaxIter - 1) {
+    if (it == 0 || (it >= 9 && it % 2 == 1) || it == kPolarMaxIter - 1) {
       DMRG_HIP(hipMemcpyAsync(hp.data() + (size_t)seen * kPolarParts, dev + (size_t)seen * kPolarParts,
                               (size_t)(it + 1 - seen) * kPolarParts * sizeof(double),
                               hipMemcpyDeviceToHost, st));
@@ -97,10 +102,10 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
           const double v = hp[(size_t)j * kPolarParts + b];
           h = (v == v) ? std::max(h, v) : INFINITY;
         }
-        // singular values s of Z move as s <- s (3 - s^2)/2: from s << 1 about
-        // log_1.5(1/s) steps, then quadratically; a zero singular value never
-        // converges, so give up when the start is too far or progress stalls.
-        if (!(h == h) || (j == 0 && h > start_tol) || (j >= 20 && h > 0.5)) return false;
+        // a zero singular value never converges: give up when the start is
+        // too far or the deviation is still O(1) after 24 steps (from
+        // s = 1e-5 the scaled iteration needs about 24)
+        if (!(h == h) || (j == 0 && h > start_tol) || (j >= 24 && h > 0.5)) return false;
         if (h <= 1e-14) {
           *Zout = Z;
           return true;

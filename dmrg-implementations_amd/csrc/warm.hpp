@@ -33,14 +33,21 @@ void overlap_right_step(hipStream_t st, const T* Tin, int r, int c, const T* Bol
 
 constexpr int kPolarMaxIter = 40;
 constexpr int kPolarParts = 256;  // workgroups of the deviation reduction
+constexpr double kPolarL0 = 1e-3;  // initial lower bound of the smallest singular value
 
-// Unitary polar factor of the n x n matrix Tin by Newton-Schulz iteration
-// Z <- Z (3I - Z^H Z)/2.  Returns false (and leaves *Zout undefined) when
+// Unitary polar factor of the n x n matrix Tin by the scaled Newton-Schulz
+// iteration Z <- (a/2) Z (3I - a^2 Z^H Z) of Chen & Chow (2014), which maps
+// every singular value s by the same odd cubic.  The scale a follows a lower
+// bound l of the smallest singular value, a = sqrt(3/(1 + l + l^2)) and
+// l <- (a/2) l (3 - a^2 l^2), from l = kPolarL0 (a -> 1, the plain
+// iteration, as l -> 1).  A singular value below the bound still grows by
+// about (3/2) a per step, so the bound is a guess that costs steps when it is
+// wrong, never convergence.  Returns false (and leaves *Zout undefined) when
 // max|T^H T - I| > start_tol, when the iteration stalls (a singular value
 // near 0) or when it does not reach max|Z^H Z - I| <= 1e-14 in kPolarMaxIter
 // steps.  Z0, Z1, P are n x n work buffers; dev holds kPolarMaxIter *
-// kPolarParts device doubles.  Synchronises the stream after the first step
-// and every third.
+// kPolarParts device doubles.  Synchronises the stream after the first step,
+// then after steps 9, 11, 13, ...
 template <class T>
 bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
                    double start_tol, T** Zout);
