@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace dmrg {
 namespace {
@@ -851,38 +852,48 @@ __global__ void k_copy_rows(const T* __restrict__ Y, int ldy, int nk, int n, T* 
 }
 
 // First-order eigenvector correction of a nearly diagonal Hermitian A
-// (Np x Np, real part of the diagonal = lambda): Q = I + E with
+// (Np x Np, real part of the diagonal = lambda): the generator E with
 // E_ij = A_ij / (lambda_j - lambda_i) for pairs the perturbation resolves
-// (|lambda_j - lambda_i| > sep |A_ij|), 0 otherwise (left to Jacobi).  E is
-// anti-Hermitian, so Q is unitary up to O(|E|^2).
-// part[blk] = sum of |E_ij|^2 over the workgroup's entries (grid of NRED).
+// (|lambda_j - lambda_i| > sep |A_ij|), 0 otherwise (left to Jacobi) and on
+// the diagonal.  E is exactly anti-Hermitian (A is Hermitian and the test is
+// symmetric), so I + E is unitary up to O(|E|^2).  One workgroup per column
+// c: colpart[c] = sum_i |E_ic|, colpart[Np + c] = sum_i |E_ic|^2.
 template <class T>
 __global__ __launch_bounds__(256) void k_oa_q(const T* __restrict__ A, T* __restrict__ Q, int N,
-                                              int Np, double sep, double* __restrict__ part) {
-  __shared__ double red[256];
-  const size_t tot = (size_t)Np * Np;
-  double e2 = 0.0;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % Np), c = (int)(e / Np);
-    T q = (r == c) ? T(1) : T(0);
+                                              int Np, double sep, double* __restrict__ colpart) {
+  __shared__ double red1[256], red2[256];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const double lc = real(A[c + (size_t)c * Np]);
+  double s1 = 0.0, s2 = 0.0;
+  for (int r = t; r < Np; r += 256) {
+    const size_t e = r + (size_t)c * Np;
+    T q = T(0);
     if (r != c && r < N && c < N) {
       const T a = A[e];
-      const double g = real(A[c + (size_t)c * Np]) - real(A[r + (size_t)r * Np]);
+      const double g = lc - real(A[r + (size_t)r * Np]);
       if (fabs(g) > sep * sqrt(abs2(a))) {
         q = a * (1.0 / g);
-        e2 += abs2(q);
+        const double q2 = abs2(q);
+        s1 += sqrt(q2);
+        s2 += q2;
       }
     }
     Q[e] = q;
   }
-  red[threadIdx.x] = e2;
+  red1[t] = s1;
+  red2[t] = s2;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    if (t < o) {
+      red1[t] += red1[t + o];
+      red2[t] += red2[t + o];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (t == 0) {
+    colpart[c] = red1[0];
+    colpart[Np + c] = red2[0];
+  }
 }
 
 // P <- (3I - P)/2 (one Newton-Schulz step's polynomial factor); with part
@@ -908,6 +919,23 @@ __global__ __launch_bounds__(256) void k_ns_half(T* __restrict__ P, int n, doubl
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// H <- a F + b I (n x n).
+template <class T>
+__global__ void k_axpy_eye(T* __restrict__ H, const T* __restrict__ F, int n, double a, double b) {
+  const size_t tot = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    H[e] = a * F[e] + T(r == c ? b : 0.0);
+  }
+}
+// X_ii += b.
+template <class T>
+__global__ void k_add_diag(T* __restrict__ X, int n, double b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) X[i + (size_t)i * n] = X[i + (size_t)i * n] + T(b);
 }
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
@@ -961,7 +989,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
   }
   double key_hi = 0.0, key_keep = 0.0;  // largest and keep-th largest diagonal entry
   double* const offpart = w.red + 2 * NRED;  // per-workgroup maxima of k_offmax
-  double* const oapart = w.red + 3 * NRED;   // per-workgroup |E|_F^2 of k_oa_q
+  static thread_local std::vector<double> col_host;  // host copy of w.colpart
   auto offmax = [&](const T* M) {
     if (keep > 0)
       hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, M, N, Np, keep,
@@ -1019,48 +1047,100 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
         if (keep > 0 && sep > 30.0) sep = 30.0;
         else break;                                  // Jacobi
       }
-      hipLaunchKernelGGL(k_oa_q<T>, dim3(NRED), dim3(256), 0, st, (const T*)w.A[0], w.Y, N, Np, sep,
-                         oapart);
-      // Make Q unitary by Newton-Schulz, Q <- Q (3I - Q^H Q)/2, which maps the
-      // deviation delta = |Q^H Q - I|_2 to (3/4) delta^2 + O(delta^3).  E is
-      // anti-Hermitian, so Q^H Q - I = E^H E and delta_0 <= |E|_F^2: a tiny E
-      // needs one step and no check.  Otherwise the Frobenius norm of
-      // Q^H Q - I (a bound on delta) is read before steps 2 and 4, and the
-      // step after a reading <= 1e-8 is the last.
-      double ep[NRED];
-      DMRG_HIP(hipMemcpyAsync(ep, oapart, sizeof(ep), hipMemcpyDeviceToHost, st));
+      hipLaunchKernelGGL(k_oa_q<T>, dim3(Np), dim3(256), 0, st, (const T*)w.A[0], w.Y, N, Np, sep,
+                         w.colpart);
+      // The unitary factor of I + E, Q2 = (I + E)(I - E^2)^(-1/2) (E is
+      // anti-Hermitian: (I + E)^H (I + E) = I - E^2).  -E^2 is positive
+      // semi-definite with |E^2|_2 = |E|_2^2 <= b = min(|E|_F^2, |E|_1^2)
+      // (|E|_1 = |E|_inf, the largest column sum, bounds |E|_2 since |E| is
+      // symmetric).  With a small b, by the binomial series
+      // (I - E^2)^(-1/2) = sum_k c_k F^k, F = E^2, c_k = C(2k, k) / 4^k,
+      // summed by Horner's rule: the series alternates, so the error of the
+      // sum up to F^m is below c_{m+1} b^(m+1), and m is the smallest order
+      // that brings it under 5e-17 (m + 1 GEMMs; m = 1 for b <= 1e-8).
+      // Otherwise Newton-Schulz on I + E, Q <- Q (3I - Q^H Q)/2, which maps
+      // the deviation delta = |Q^H Q - I|_2 to (3/4) delta^2 + O(delta^3),
+      // with delta_0 <= b: the Frobenius norm of Q^H Q - I is read before
+      // steps 2 and 4, and the step after a reading <= 1e-8 is the last.
+      std::vector<double>& cp = col_host;
+      cp.resize(2 * (size_t)Np);
+      DMRG_HIP(hipMemcpyAsync(cp.data(), w.colpart, 2 * (size_t)Np * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
       DMRG_HIP(hipStreamSynchronize(st));
-      double e2 = 0.0;
-      for (double v : ep) e2 += v;
-      if (!(e2 == e2)) break;
-      double* const nspart = w.red + 2 * NRED;  // free while no offmax is pending
-      const bool tiny = e2 <= 1e-8;
-      bool unitary = false;
-      for (int ns = 0; ns < (tiny ? 1 : 4) && !unitary; ++ns) {
-        const bool check = !tiny && (ns == 1 || ns == 3);
-        gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), w.Y, Np, w.Y, Np, T(0), w.A[1], Np, tl);
-        hipLaunchKernelGGL(k_ns_half<T>, dim3(NRED), dim3(256), 0, st, w.A[1], Np,
-                           check ? nspart : nullptr);
-        gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, w.A[1], Np, T(0), w.V[1], Np, tl);
-        std::swap(w.Y, w.V[1]);
-        if (tiny) unitary = true;
-        if (check) {
-          double dp[NRED];
-          DMRG_HIP(hipMemcpyAsync(dp, nspart, sizeof(dp), hipMemcpyDeviceToHost, st));
-          DMRG_HIP(hipStreamSynchronize(st));
-          double d2 = 0.0;
-          for (double v : dp) d2 += v;
-          if (!(d2 == d2)) break;
-          unitary = d2 <= 1e-16;  // |Q^H Q - I|_F <= 1e-8 before this step
-        }
+      double e2 = 0.0, c1 = 0.0;
+      for (int c = 0; c < Np; ++c) {
+        c1 = std::max(c1, cp[c]);
+        e2 += cp[Np + c];
       }
-      if (!unitary) break;  // leave it to Jacobi (A, V unchanged)
-      std::swap(w.Y, w.V[1]);  // Q2 -> V[1]
-      gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.A[0], Np, w.V[1], Np, T(0), w.Y, Np, tl);  // A Q2
-      gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), w.V[1], Np, w.Y, Np, T(0), w.A[1], Np, tl);  // Q2^H A Q2
-      gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.V[0], Np, w.V[1], Np, T(0), w.Y, Np, tl);  // V Q2
-      std::swap(w.A[0], w.A[1]);
-      std::swap(w.V[0], w.Y);
+      if (!(e2 == e2) || !(c1 == c1)) break;
+      const double bnd = std::min(e2, c1 * c1);
+      constexpr double cser[8] = {1.0, 0.5, 0.375, 0.3125, 0.2734375, 0.24609375, 0.2255859375,
+                                  0.20947265625};
+      int m = 1;
+      while (m < 6 && cser[m + 1] * std::pow(bnd, m + 1) > 5e-17) ++m;
+      T* freeb[4] = {w.A[1], w.V[1], w.W, w.Y};  // Y holds E
+      T* Q2 = nullptr;
+      if (cser[m + 1] * std::pow(bnd, m + 1) <= 5e-17) {
+        T* F = freeb[0];
+        T* H = freeb[1];
+        T* G = freeb[2];
+        gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, w.Y, Np, T(0), F, Np, tl);  // F = E^2
+        hipLaunchKernelGGL(k_axpy_eye<T>, dim3(grid_for(tot)), dim3(256), 0, st, H, (const T*)F,
+                           Np, cser[m], cser[m - 1]);
+        for (int j = m - 2; j >= 0; --j) {
+          gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), F, Np, H, Np, T(0), G, Np, tl);
+          hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, G, Np, cser[j]);
+          std::swap(H, G);
+        }
+        hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, w.Y, Np, 1.0);
+        DMRG_LAUNCH_CHECK();
+        Q2 = G;
+        gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, H, Np, T(0), Q2, Np, tl);  // (I + E) S
+      } else {
+        hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, w.Y, Np, 1.0);
+        DMRG_LAUNCH_CHECK();
+        double* const nspart = w.red + 2 * NRED;  // free while no offmax is pending
+        T* Z = w.Y;
+        T* P = freeb[0];
+        T* Zn = freeb[1];
+        bool unitary = false;
+        for (int ns = 0; ns < 4 && !unitary; ++ns) {
+          const bool check = ns == 1 || ns == 3;
+          gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), Z, Np, Z, Np, T(0), P, Np, tl);
+          hipLaunchKernelGGL(k_ns_half<T>, dim3(NRED), dim3(256), 0, st, P, Np,
+                             check ? nspart : nullptr);
+          gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), Z, Np, P, Np, T(0), Zn, Np, tl);
+          std::swap(Z, Zn);
+          if (check) {
+            double dp[NRED];
+            DMRG_HIP(hipMemcpyAsync(dp, nspart, sizeof(dp), hipMemcpyDeviceToHost, st));
+            DMRG_HIP(hipStreamSynchronize(st));
+            double d2 = 0.0;
+            for (double v : dp) d2 += v;
+            if (!(d2 == d2)) break;
+            unitary = d2 <= 1e-16;  // |Q^H Q - I|_F <= 1e-8 before this step
+          }
+        }
+        if (!unitary) break;  // leave it to Jacobi (A, V unchanged)
+        Q2 = Z;
+      }
+      // The three other work buffers: X1 = A Q2, X2 = Q2^H X1 (the new A),
+      // X3 = V Q2 (the new V).
+      T* xs[3];
+      int nx = 0;
+      for (T* bptr : freeb)
+        if (bptr != Q2) xs[nx++] = bptr;
+      gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.A[0], Np, Q2, Np, T(0), xs[0], Np, tl);
+      gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), Q2, Np, xs[0], Np, T(0), xs[1], Np, tl);
+      gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.V[0], Np, Q2, Np, T(0), xs[2], Np, tl);
+      T* const oldA = w.A[0];
+      T* const oldV = w.V[0];
+      w.A[0] = xs[1];
+      w.V[0] = xs[2];
+      w.A[1] = oldA;
+      w.V[1] = oldV;
+      w.W = Q2;
+      w.Y = xs[0];
       hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[0], w.V[0], N, Np, 0);
       DMRG_LAUNCH_CHECK();
       ++cnt.oa;
@@ -1122,12 +1202,14 @@ void SvdWork<T>::alloc(int cap_n) {
     DMRG_HIP(hipMalloc(&V[i], mat));
   }
   DMRG_HIP(hipMalloc(&Y, mat));
+  DMRG_HIP(hipMalloc(&W, mat));
   DMRG_HIP(hipMalloc(&Q, (size_t)(cap / JP) * JP * JP * sizeof(T)));
   DMRG_HIP(hipMalloc(&qflag, (cap / JP) * sizeof(int)));
   DMRG_HIP(hipMalloc(&perm, cap * sizeof(int)));
   DMRG_HIP(hipMalloc(&rank, cap * sizeof(int)));
   DMRG_HIP(hipMalloc(&gdiag, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&red, 4 * NRED * sizeof(double)));
+  DMRG_HIP(hipMalloc(&colpart, 2 * (size_t)cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&sigma, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&resid, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&scal, 4 * sizeof(double)));
@@ -1141,16 +1223,18 @@ void SvdWork<T>::release() {
     if (V[i]) (void)hipFree(V[i]);
     A[i] = V[i] = nullptr;
   }
-  void* ps[] = {Y, Q, qflag, perm, rank, gdiag, red, sigma, resid, scal, newk};
+  void* ps[] = {Y, W, Q, qflag, perm, rank, gdiag, red, colpart, sigma, resid, scal, newk};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   Y = nullptr;
+  W = nullptr;
   Q = nullptr;
   qflag = nullptr;
   perm = nullptr;
   rank = nullptr;
   gdiag = nullptr;
   red = nullptr;
+  colpart = nullptr;
   sigma = nullptr;
   resid = nullptr;
   scal = nullptr;

@@ -41,12 +41,14 @@ struct SvdWork {
   T* A[2] = {nullptr, nullptr};  // density matrix, ping-pong (capN^2 each)
   T* V[2] = {nullptr, nullptr};  // accumulated eigenvectors
   T* Y = nullptr;                // staging for the gathered / product factor
+  T* W = nullptr;                // work matrix of the first-order steps
   T* Q = nullptr;                // per-pair 32x32 rotations of one round
   int* qflag = nullptr;          // per pair: 1 if its Q is not the identity
   int* perm = nullptr;           // sort permutation
   int* rank = nullptr;           // inverse of perm (position of each index)
   double* gdiag = nullptr;       // squared norms of the k product vectors
   double* red = nullptr;         // reduction partials / small results (1024 doubles)
+  double* colpart = nullptr;     // per-column sums of the first-order step (2 cap doubles)
   double* sigma = nullptr;       // singular values in output order
   double* resid = nullptr;       // first-pass QR residuals
   double* scal = nullptr;        // [0] |rho|_F, [1] discarded weight, [2] trace(rho)
