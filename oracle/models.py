@@ -23,6 +23,12 @@ Pinning
 * ``dense_*``: pinned by those same known answers (Heisenberg L=4/8,
   Josephson n_max=1 L=4/6) and by closed forms (two-site singlet -3/4,
   ferromagnet (L-1)/4, TFIM h=0 / J=0 limits).
+* ``spin_chain_mpo`` / ``ladder_mpo`` and ``dense_spin_chain`` /
+  ``dense_ladder``: MPO against Kronecker form element-wise; ground energies
+  against the reference's known answers (J1-J2 at the Majumdar-Ghosh point,
+  -3L/8: test_dmrg2_gpu.cpp:344-346; J1-J2-J3: :390-394; ladder: :442-447),
+  the Majumdar-Ghosh dimer closed form and J2 = J3 = 0 reducing to
+  ``dense_heisenberg``.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -72,6 +78,62 @@ def random_field_heisenberg_mpo(L, h, J=1.0):
             _set_block(W, 0, 4, h[i] * _SZ)
         if first and last:  # L == 1: only the on-site term survives
             pass
+        Ws.append(W)
+    return Ws
+
+
+def spin_chain_mpo(L, Js):
+    """H = sum_{r=1..R} J_r sum_i S_i.S_{i+r}, open chain, R = len(Js).
+
+    Finite-automaton MPO: state 0 = nothing placed; states (r, a, j) for the
+    range r, the opening operator a in (S+, S-, Sz) and j = 0..r-1 identities
+    placed since; state D-1 = complete.  D = 2 + 3R(R+1)/2 (11 for J1-J2,
+    20 for J1-J2-J3; gpu-rocm/common/challenge_mpos.h:18-51,
+    test_dmrg2_gpu.cpp:329-418 use the same bond dimensions)."""
+    R = len(Js)
+    D = 2 + 3 * R * (R + 1) // 2
+    opens = (_SP, _SM, _SZ)
+    closes = (0.5 * _SM, 0.5 * _SP, _SZ)
+    Ws = []
+    for _ in range(L):
+        W = np.zeros((D, 2, 2, D))
+        _set_block(W, 0, 0, _ID2)
+        _set_block(W, D - 1, D - 1, _ID2)
+        base = 1
+        for r in range(1, R + 1):
+            for a in range(3):
+                s0 = base + a * r
+                _set_block(W, 0, s0, opens[a])
+                for j in range(r - 1):
+                    _set_block(W, s0 + j, s0 + j + 1, _ID2)
+                _set_block(W, s0 + r - 1, D - 1, Js[r - 1] * closes[a])
+            base += 3 * r
+        Ws.append(W)
+    return Ws
+
+
+def ladder_mpo(L, J_leg=1.0, J_rung=1.0):
+    """Two-leg Heisenberg ladder, one rung per site (d = 4, s = 2 s1 + s2),
+    H = J_leg sum_i (S1_i.S1_{i+1} + S2_i.S2_{i+1}) + J_rung sum_i S1_i.S2_i,
+    D = 8: 0 start, 1-3 / 4-6 leg 1 / leg 2 opened by S+, S-, Sz, 7 done;
+    the rung term on-site in block (0, 7) (gpu-rocm/common/challenge_mpos.h:126-170)."""
+    I2 = _ID2
+    S1 = [np.kron(o, I2) for o in (_SP, _SM, _SZ)]
+    S2 = [np.kron(I2, o) for o in (_SP, _SM, _SZ)]
+    rung = J_rung * (0.5 * (np.kron(_SP, _SM) + np.kron(_SM, _SP)) + np.kron(_SZ, _SZ))
+    partner, scale = (1, 0, 2), (0.5, 0.5, 1.0)
+    D = 8
+    Ws = []
+    for _ in range(L):
+        W = np.zeros((D, 4, 4, D))
+        _set_block(W, 0, 0, np.eye(4))
+        _set_block(W, D - 1, D - 1, np.eye(4))
+        _set_block(W, 0, D - 1, rung)
+        for leg, S in enumerate((S1, S2)):
+            for a in range(3):
+                st = 1 + 3 * leg + a
+                _set_block(W, 0, st, S[a])
+                _set_block(W, st, D - 1, J_leg * scale[a] * S[partner[a]])
         Ws.append(W)
     return Ws
 
@@ -180,6 +242,30 @@ def dense_heisenberg(L, J=1.0, h=None):
             if h[i] != 0.0:
                 H = H + h[i] * _site_op(L, i, _SZ, 2)
     return H
+
+
+def _dense_ss(n, pairs):
+    """sum over (i, j, c) in pairs of c S_i.S_j on n spins (Kronecker products)."""
+    H = sp.csr_matrix((2 ** n, 2 ** n))
+    for i, j, c in pairs:
+        for a, b, f in ((_SP, _SM, 0.5), (_SM, _SP, 0.5), (_SZ, _SZ, 1.0)):
+            ops = [np.eye(2)] * n
+            ops = list(ops)
+            ops[i], ops[j] = a, b
+            H = H + (c * f) * _kron_chain(ops)
+    return H
+
+
+def dense_spin_chain(L, Js):
+    """H = sum_r J_r sum_i S_i.S_{i+r} from the operator definitions."""
+    return _dense_ss(L, [(i, i + r, J) for r, J in enumerate(Js, start=1) for i in range(L - r)])
+
+
+def dense_ladder(L, J_leg=1.0, J_rung=1.0):
+    """2L spins, rung i = spins (2i, 2i+1) (leg 1 first, matching s = 2 s1 + s2)."""
+    pairs = [(2 * i, 2 * i + 1, J_rung) for i in range(L)]
+    pairs += [(2 * i + leg, 2 * i + 2 + leg, J_leg) for i in range(L - 1) for leg in (0, 1)]
+    return _dense_ss(2 * L, pairs)
 
 
 def dense_tfim(L, J=1.0, hx=1.0):

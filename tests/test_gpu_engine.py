@@ -250,3 +250,27 @@ def test_reference_bin_inputs(dmrg_amd, case):
     assert abs(n2 - 1) < 1e-10
     if case == "heisenberg":
         assert abs(E - (-5.142090632840533)) < 1e-10
+
+
+@pytest.mark.parametrize("model,L,args,E_ref", [
+    ("chain", 8, [1.0, 0.5], -3.0),                        # Majumdar-Ghosh, -3L/8
+    ("chain", 8, [1.0, 0.4, 0.2], -3.1205086380),          # test_dmrg2_gpu.cpp:390-394
+    ("ladder", 4, [1.0, 1.0], -4.2930664566945656),        # :442-447
+    ("chain", 10, [0.8, -0.3, 0.45], None),                # ED only
+])
+def test_frustrated_chain_and_ladder(dmrg_amd, model, L, args, E_ref):
+    """The J1-J2(-J3) chains (D = 11, 20) and the d = 4 ladder (D = 8)
+    against exact diagonalisation and the reference's known answers."""
+    if model == "ladder":
+        Ws, d, H = models.ladder_mpo(L, *args), 4, models.dense_ladder(L, *args)
+    else:
+        Ws, d, H = models.spin_chain_mpo(L, args), 2, models.dense_spin_chain(L, args)
+    E_ed = models.ground_energy(H)
+    eng = dmrg_amd.DMRG(L, d, 64, Ws[0].shape[0])
+    eng.set_mpo(Ws)
+    eng.initialize_mps_random(seed=11, scale=0.1)
+    E = eng.run(16)
+    eng.close()
+    assert abs(E - E_ed) < 1e-9
+    if E_ref is not None:
+        assert abs(E - E_ref) < 1e-8

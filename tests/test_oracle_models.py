@@ -107,3 +107,69 @@ def test_fixture_heisenberg_initial_mps_left_canonical():
         cL, d, cR = A.shape
         M = A.reshape(cL * d, cR, order="F")
         assert np.abs(M.T @ M - np.eye(cR)).max() < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# Frustrated chains and the two-leg ladder (the CLI's --j1j2, --j1j2j3 and
+# --ladder models; gpu-rocm/dmrg2-gpu/src/test_dmrg2_gpu.cpp:329-470).
+
+@pytest.mark.parametrize("L,Js", [(5, [0.9, -0.37]), (6, [1.0, 0.4, 0.2]), (6, [0.7, -0.3, 0.55])])
+def test_spin_chain_mpo_is_the_kronecker_hamiltonian(L, Js):
+    H = models.mpo_to_dense(models.spin_chain_mpo(L, Js))
+    assert np.abs(H - models.dense_spin_chain(L, Js).toarray()).max() < 1e-14
+    assert models.spin_chain_mpo(L, Js)[0].shape[0] == 2 + 3 * len(Js) * (len(Js) + 1) // 2
+
+
+def test_spin_chain_range_one_is_heisenberg():
+    # J2 = J3 = 0 reduces to the nearest-neighbour chain and its MPO
+    for L in (4, 7):
+        H = models.dense_spin_chain(L, [1.0, 0.0, 0.0]).toarray()
+        assert np.abs(H - models.dense_heisenberg(L).toarray()).max() == 0.0
+        Hm = models.mpo_to_dense(models.spin_chain_mpo(L, [1.0]))
+        assert np.abs(Hm - models.mpo_to_dense(models.heisenberg_mpo(L))).max() < 1e-15
+
+
+@pytest.mark.parametrize("L", [4, 6, 8])
+def test_j1j2_majumdar_ghosh_closed_form(L):
+    """At J2 = J1/2 a product of nearest-neighbour singlets is an exact ground
+    state with E = -3L/8 J1 (even L); the reference tabulates the same values
+    (test_dmrg2_gpu.cpp:344-346)."""
+    E = models.ground_energy(models.dense_spin_chain(L, [1.0, 0.5]))
+    assert abs(E - (-3.0 * L / 8.0)) < 1e-12
+    # the dimer state itself: singlets on (0,1), (2,3), ...
+    s = np.array([0.0, 1.0, -1.0, 0.0]) / np.sqrt(2.0)   # (|ud> - |du>)/sqrt 2
+    psi = s
+    for _ in range(L // 2 - 1):
+        psi = np.kron(psi, s)
+    H = models.dense_spin_chain(L, [1.0, 0.5])
+    assert abs(psi @ (H @ psi) + 3.0 * L / 8.0) < 1e-12
+    assert np.abs(H @ psi + 3.0 * L / 8.0 * psi).max() < 1e-12
+
+
+@pytest.mark.parametrize("L,Js,E_ref", [(6, [1.0, 0.4, 0.2], -2.3229239173),
+                                        (8, [1.0, 0.4, 0.2], -3.1205086380),
+                                        (8, [1.0, 0.5, 0.25], -3.0690408638)])
+def test_j1j2j3_known_answers(L, Js, E_ref):
+    """test_dmrg2_gpu.cpp:390-394 (quoted to 10 decimals)."""
+    assert abs(models.ground_energy(models.dense_spin_chain(L, Js)) - E_ref) < 1e-9
+
+
+@pytest.mark.parametrize("L", [2, 3])
+def test_ladder_mpo_is_the_kronecker_hamiltonian(L):
+    H = models.mpo_to_dense(models.ladder_mpo(L, 0.8, 1.3))
+    assert np.abs(H - models.dense_ladder(L, 0.8, 1.3).toarray()).max() < 1e-14
+
+
+@pytest.mark.parametrize("L,E_ref", [(2, -2.0), (3, -3.1293852415718166), (4, -4.2930664566945656)])
+def test_ladder_known_answers(L, E_ref):
+    """test_dmrg2_gpu.cpp:442-447; L = 2 rungs is the 4-spin ring, E = -2."""
+    assert abs(models.ground_energy(models.dense_ladder(L)) - E_ref) < 1e-9
+
+
+def test_ladder_decoupled_rungs():
+    # J_leg = 0: independent rung singlets, -3/4 J_rung each
+    for L in (2, 3):
+        E = models.ground_energy(models.dense_ladder(L, J_leg=0.0, J_rung=1.7))
+        assert abs(E - (-0.75 * 1.7 * L)) < 1e-12
+        Em = models.ground_energy(models.mpo_to_dense(models.ladder_mpo(L, 0.0, 1.7)))
+        assert abs(Em - E) < 1e-12
