@@ -63,7 +63,7 @@ template <class T, bool CA, bool CB, bool KFA, bool KFB, int BM, int BN, int BK,
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A,
                                               AA aa, long long sA, const T* __restrict__ B, AB ab,
                                               long long sB, T beta, T* __restrict__ C, AC ac,
-                                              long long sC) {
+                                              long long sC, int herm) {
   using E = typename GT<T>::E;
   using V4 = typename GT<T>::V4;
   constexpr int P = GT<T>::P;
@@ -74,6 +74,9 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
   __shared__ E As[2][P][BK][PADM];
   __shared__ E Bs[2][P][BK][PADN];
 
+  // Hermitian product (M == N, BM == BN, beta == 0): only the tiles on and
+  // above the diagonal are computed; each writes its conjugate mirror too.
+  if (herm && blockIdx.x > blockIdx.y) return;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -222,6 +225,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
           long long idx = ac(gm, gn);
           if (use_beta) v = v + beta * C[idx];
           C[idx] = v;
+          if (herm && blockIdx.x != blockIdx.y) C[ac(gn, gm)] = conj_v(v);
         }
       }
 }
@@ -229,11 +233,11 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 template <class T, bool CA, bool CB, bool KFA, bool KFB, class AA, class AB, class AC>
 void launch(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* A, AA aa,
             long long sA, const T* B, AB ab, long long sB, T beta, T* C, AC ac, long long sC,
-            int batch) {
+            int batch, int herm = 0) {
 #define DMRG_GEMM_LAUNCH(BM_, BN_, BK_, PF_)                                                \
   hipLaunchKernelGGL((k_gemm<T, CA, CB, KFA, KFB, BM_, BN_, BK_, PF_, AA, AB, AC>),         \
                      dim3(cdiv(M, BM_), cdiv(N, BN_), batch), dim3(256), 0, st, M, N, K,    \
-                     alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC)
+                     alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC, herm)
   // 32x32 tiles serve grids that cannot fill the chip with 64x64 ones; a deep
   // k-step and two register stages then hide the load latency that the low
   // tile count per CU leaves exposed (real types; complex already issues four
@@ -251,15 +255,16 @@ void launch(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* 
 template <class T, bool TA, bool TB, bool CA, bool CB>
 void plain4(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* A, int lda,
             long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc, long long sC,
-            int batch) {
+            int batch, int herm) {
   launch<T, CA, CB, TA, !TB>(st, tl, M, N, K, alpha, A, AddrPlain<TA>{lda}, sA, B,
-                             AddrPlain<TB>{ldb}, sB, beta, C, AddrPlain<false>{ldc}, sC, batch);
+                             AddrPlain<TB>{ldb}, sB, beta, C, AddrPlain<false>{ldc}, sC, batch,
+                             herm);
 }
 
 template <class T>
 void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T alpha, const T* A,
            int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc,
-           long long sC, int batch) {
+           long long sC, int batch, int herm = 0) {
   DMRG_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "gemm: negative size");
   DMRG_REQUIRE(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "gemm: bad op");
   if (M == 0 || N == 0 || batch == 0) return;
@@ -267,7 +272,8 @@ void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T
   bool ca = op_conj(opA), cb = op_conj(opB);
   if (!is_cplx<T>::value) ca = cb = false;
 #define DMRG_P4(TA_, TB_, CA_, CB_) \
-  plain4<T, TA_, TB_, CA_, CB_>(st, tl, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch)
+  plain4<T, TA_, TB_, CA_, CB_>(st, tl, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, \
+                                batch, herm)
   if (!ca && !cb) {
     if (!ta && !tb) DMRG_P4(false, false, false, false);
     else if (!ta && tb) DMRG_P4(false, true, false, false);
@@ -556,6 +562,13 @@ void gemm(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const 
 }
 
 template <class T>
+void gemm_herm(hipStream_t st, int opA, int opB, int N, int K, T alpha, const T* A, int lda,
+               const T* B, int ldb, T* C, int ldc, GemmTile tile) {
+  DMRG_REQUIRE(tile.bm == tile.bn, "gemm_herm: square tiles only");
+  plain<T>(st, opA, opB, tile, N, N, K, alpha, A, lda, 0, B, ldb, 0, T(0), C, ldc, 0, 1, 1);
+}
+
+template <class T>
 void gemm_strided(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const T* A,
                   int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc,
                   long long sC, int batch, GemmTile tile) {
@@ -606,6 +619,8 @@ void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2
 #define DMRG_INST(T)                                                                         \
   template void gemm<T>(hipStream_t, int, int, int, int, int, T, const T*, int, const T*, int, \
                         T, T*, int, GemmTile);                                                 \
+  template void gemm_herm<T>(hipStream_t, int, int, int, int, T, const T*, int, const T*, int,  \
+                             T*, int, GemmTile);                                                \
   template void gemm_strided<T>(hipStream_t, int, int, int, int, int, T, const T*, int,        \
                                 long long, const T*, int, long long, T, T*, int, long long,    \
                                 int, GemmTile);                                                \

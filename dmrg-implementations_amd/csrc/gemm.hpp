@@ -41,6 +41,14 @@ void gemm(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const 
           const T* B, int ldb, T beta, T* C, int ldc, GemmTile tile = {64, 64});
 
 // Strided batch: batch b uses A + b*sA, B + b*sB, C + b*sC.
+// C = alpha op(A) op(B) for a product known to be Hermitian (N x N; e.g.
+// X^H X, or X^H M X with M Hermitian): only the square tiles on and above the
+// diagonal are computed, each also writing its conjugate transpose below.
+// The result's two triangles are then exact conjugates off the diagonal tiles.
+template <class T>
+void gemm_herm(hipStream_t st, int opA, int opB, int N, int K, T alpha, const T* A, int lda,
+               const T* B, int ldb, T* C, int ldc, GemmTile tile = {64, 64});
+
 template <class T>
 void gemm_strided(hipStream_t st, int opA, int opB, int M, int N, int K, T alpha, const T* A,
                   int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C,

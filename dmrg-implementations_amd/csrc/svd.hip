@@ -155,17 +155,31 @@ __global__ __launch_bounds__(256) void k_rank_diag(const T* __restrict__ A, int 
     dg[i] = (v == v) ? v : -INFINITY;
   }
   __syncthreads();
-  for (int i = blockIdx.x * 256 + t; i < Np; i += gridDim.x * 256) {
-    if (i >= N) { rank[i] = i; continue; }
-    const double di = dg[i];
+  // Four lanes per entry (a quad, all with the same i): lane q counts the
+  // entries j = q, q + 4, ...; the quad sums by shuffles.  Grid: cdiv(Np, 64).
+  const int q = t & 3;
+  for (int i = blockIdx.x * 64 + (t >> 2); i < Np; i += gridDim.x * 64) {
     int r = 0;
-    for (int j = 0; j < N; ++j) {
-      const double dj = dg[j];
-      r += (dj > di) || (dj == di && j < i);
+    double di = 0.0;
+    if (i < N) {
+      di = dg[i];
+#pragma unroll 8
+      for (int j = q; j < N; j += 4) {
+        const double dj = dg[j];
+        r += (dj > di) || (dj == di && j < i);
+      }
     }
-    rank[i] = r;
-    if (r == 0) keys[0] = di;
-    if (r == keep - 1) keys[1] = di;
+    r += __shfl_xor(r, 1);
+    r += __shfl_xor(r, 2);
+    if (q == 0) {
+      if (i >= N) {
+        rank[i] = i;
+      } else {
+        rank[i] = r;
+        if (r == 0) keys[0] = di;
+        if (r == keep - 1) keys[1] = di;
+      }
+    }
   }
 }
 
@@ -992,7 +1006,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
   static thread_local std::vector<double> col_host;  // host copy of w.colpart
   auto offmax = [&](const T* M) {
     if (keep > 0)
-      hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, M, N, Np, keep,
+      hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 64)), dim3(256), 0, st, M, N, Np, keep,
                          w.rank, w.red);
     hipLaunchKernelGGL(k_offmax<T>, dim3(NRED), dim3(256), 0, st, M, N, Np,
                        keep > 0 ? (const int*)w.rank : nullptr, keep, offpart);
@@ -1084,11 +1098,11 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
         T* F = freeb[0];
         T* H = freeb[1];
         T* G = freeb[2];
-        gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, w.Y, Np, T(0), F, Np, tl);  // F = E^2
+        gemm_herm<T>(st, OP_N, OP_N, Np, Np, T(1), w.Y, Np, w.Y, Np, F, Np, tl);  // F = E^2
         hipLaunchKernelGGL(k_axpy_eye<T>, dim3(grid_for(tot)), dim3(256), 0, st, H, (const T*)F,
                            Np, cser[m], cser[m - 1]);
         for (int j = m - 2; j >= 0; --j) {
-          gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), F, Np, H, Np, T(0), G, Np, tl);
+          gemm_herm<T>(st, OP_N, OP_N, Np, Np, T(1), F, Np, H, Np, G, Np, tl);  // F, H commute
           hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, G, Np, cser[j]);
           std::swap(H, G);
         }
@@ -1106,7 +1120,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
         bool unitary = false;
         for (int ns = 0; ns < 4 && !unitary; ++ns) {
           const bool check = ns == 1 || ns == 3;
-          gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), Z, Np, Z, Np, T(0), P, Np, tl);
+          gemm_herm<T>(st, OP_H, OP_N, Np, Np, T(1), Z, Np, Z, Np, P, Np, tl);
           hipLaunchKernelGGL(k_ns_half<T>, dim3(NRED), dim3(256), 0, st, P, Np,
                              check ? nspart : nullptr);
           gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), Z, Np, P, Np, T(0), Zn, Np, tl);
@@ -1131,7 +1145,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
       for (T* bptr : freeb)
         if (bptr != Q2) xs[nx++] = bptr;
       gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.A[0], Np, Q2, Np, T(0), xs[0], Np, tl);
-      gemm<T>(st, OP_H, OP_N, Np, Np, Np, T(1), Q2, Np, xs[0], Np, T(0), xs[1], Np, tl);
+      gemm_herm<T>(st, OP_H, OP_N, Np, Np, T(1), Q2, Np, xs[0], Np, xs[1], Np, tl);
       gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.V[0], Np, Q2, Np, T(0), xs[2], Np, tl);
       T* const oldA = w.A[0];
       T* const oldV = w.V[0];
@@ -1254,18 +1268,18 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
   const int k = std::min(std::min(m, n), kmax);
   // density matrix on the side of the orthonormal factor
   if (!scale_left)
-    gemm<T>(st, OP_N, OP_H, m, m, n, T(1), A, lda, A, lda, T(0), w.A[0], Np,
-            gemm_pick_tile(m, m, n, is_cplx<T>::value));
+    gemm_herm<T>(st, OP_N, OP_H, m, n, T(1), A, lda, A, lda, w.A[0], Np,
+                 gemm_pick_tile(m, m, n, is_cplx<T>::value));
   else
-    gemm<T>(st, OP_H, OP_N, n, n, m, T(1), A, lda, A, lda, T(0), w.A[0], Np,
-            gemm_pick_tile(n, n, m, is_cplx<T>::value));
+    gemm_herm<T>(st, OP_H, OP_N, n, m, T(1), A, lda, A, lda, w.A[0], Np,
+                 gemm_pick_tile(n, n, m, is_cplx<T>::value));
   if (warm) {
     // start from the caller's orthonormal basis X: rho <- X^H rho X, V = X
     hipLaunchKernelGGL(k_set_basis<T>, dim3(grid_for((size_t)Np * Np)), dim3(256), 0, st, w.V[0],
                        warm, N, Np);
     const GemmTile tl = gemm_pick_tile(N, N, N, is_cplx<T>::value);
     gemm<T>(st, OP_N, OP_N, N, N, N, T(1), w.A[0], Np, w.V[0], Np, T(0), w.A[1], Np, tl);
-    gemm<T>(st, OP_H, OP_N, N, N, N, T(1), w.V[0], Np, w.A[1], Np, T(0), w.A[0], Np, tl);
+    gemm_herm<T>(st, OP_H, OP_N, N, N, T(1), w.V[0], Np, w.A[1], Np, w.A[0], Np, tl);
   }
   SvdResult res;
   res.N = N;
@@ -1285,16 +1299,16 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
                        (const T*)w.V[cur], Np, (const int*)w.perm, m, k, 0, left, ldl);
     gemm<T>(st, OP_H, OP_N, k, n, m, T(1), left, ldl, A, lda, T(0), w.Y, k,
             gemm_pick_tile(k, n, m, is_cplx<T>::value));
-    gemm<T>(st, OP_N, OP_H, k, k, n, T(1), w.Y, k, w.Y, k, T(0), w.A[1 - cur], k,
-            gemm_pick_tile(k, k, n, is_cplx<T>::value));
+    gemm_herm<T>(st, OP_N, OP_H, k, n, T(1), w.Y, k, w.Y, k, w.A[1 - cur], k,
+                 gemm_pick_tile(k, k, n, is_cplx<T>::value));
   } else {
     // V_k -> Y (n x k); U S = theta V_k -> left; G = left^H left
     hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)n * k)), dim3(256), 0, st,
                        (const T*)w.V[cur], Np, (const int*)w.perm, n, k, 0, w.Y, n);
     gemm<T>(st, OP_N, OP_N, m, k, n, T(1), A, lda, w.Y, n, T(0), left, ldl,
             gemm_pick_tile(m, k, n, is_cplx<T>::value));
-    gemm<T>(st, OP_H, OP_N, k, k, m, T(1), left, ldl, left, ldl, T(0), w.A[1 - cur], k,
-            gemm_pick_tile(k, k, m, is_cplx<T>::value));
+    gemm_herm<T>(st, OP_H, OP_N, k, m, T(1), left, ldl, left, ldl, w.A[1 - cur], k,
+                 gemm_pick_tile(k, k, m, is_cplx<T>::value));
   }
   // sigma_j: residual of product vector j after projecting out the earlier
   // clean pivots, computed on the vectors themselves (absolute accuracy

@@ -87,7 +87,7 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
   for (int it = 0; it < kPolarMaxIter; ++it) {
     const double a = l < 1.0 ? std::sqrt(3.0 / (1.0 + l + l * l)) : 1.0;
     l = std::min(1.0, 0.5 * a * l * (3.0 - a * a * l * l));
-    gemm<T>(st, OP_H, OP_N, n, n, n, T(1), Z, n, Z, n, T(0), P, n, gemm_pick_tile(n, n, n, cx));
+    gemm_herm<T>(st, OP_H, OP_N, n, n, T(1), Z, n, Z, n, P, n, gemm_pick_tile(n, n, n, cx));
     hipLaunchKernelGGL(k_ns_prep<T>, dim3(kPolarParts), dim3(256), 0, st, P, n, a,
                        dev + (size_t)it * kPolarParts);
     DMRG_LAUNCH_CHECK();
