@@ -131,7 +131,7 @@ def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
     Ws = workload_mpo(wl)
     eng = dmrg_amd.DMRG(wl["L"], wl.get("d", 2), wl["chi"], wl.get("D", 5),
                         dtype=wl.get("dtype", "f64"), tol=0.0, krylov_max=wl["krylov"],
-                        precision=precision, svd_tol=svd_tol, svd_tol_mixed=svd_tol)
+                        precision=precision, svd_tol=svd_tol, svd_tol_mixed=svd_tol, profile=1)
     eng.set_mpo(Ws)
     eng.initialize_mps_random(seed=wl["seed"], scale=0.1)
     warm_sweeps = []

@@ -39,9 +39,9 @@ double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int s
   int niter = max_iter, it = 0, nmv = 0;
   for (it = 0; it < max_iter; ++it) {
     T* vi = V + (size_t)it * n;
-    DMRG_HIP(hipEventRecord(ev[2 * it], st));
+    if (timed) DMRG_HIP(hipEventRecord(ev[2 * it], st));
     matvec(vi, work);
-    DMRG_HIP(hipEventRecord(ev[2 * it + 1], st));
+    if (timed) DMRG_HIP(hipEventRecord(ev[2 * it + 1], st));
     ++nmv;
     lz_alpha<T>(st, *lz, vi, work, n, it);
     lz_sub_dots<T>(st, *lz, work, V, n, it);
@@ -63,7 +63,7 @@ double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int s
   DMRG_HIP(hipMemcpyAsync(h_scal, lz->scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
   DMRG_HIP(hipStreamSynchronize(st));
   double ms = 0.0;
-  for (int j = 0; j < nmv; ++j) {
+  for (int j = 0; timed && j < nmv; ++j) {
     float t = 0.f;
     DMRG_HIP(hipEventElapsedTime(&t, ev[2 * j], ev[2 * j + 1]));
     ms += t;
@@ -184,6 +184,7 @@ void Engine<T>::alloc_all() {
   DMRG_HIP(hipHostMalloc((void**)&h_status, 4 * sizeof(int), hipHostMallocDefault));
   DMRG_HIP(hipHostMalloc((void**)&h_scal, 8 * sizeof(double), hipHostMallocDefault));
   runner.lz = &lz;
+  runner.timed = opt.profile != 0;
   runner.V = V;
   runner.work = work;
   runner.h_status = h_status;

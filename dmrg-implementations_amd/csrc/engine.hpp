@@ -72,6 +72,7 @@ struct LanczosRunner {
   int* h_status = nullptr;   // pinned [4]
   double* h_scal = nullptr;  // pinned [8]
   std::vector<hipEvent_t> ev;  // 2 per matvec
+  bool timed = false;          // record the events (option profile): H_eff time in the stats
   double beta_tol = 1e-12, eig_tol = 1e-12, eig_rel = 0.0;
   int last_niter = 0, last_matvecs = 0;
   double last_heff_ms = 0;

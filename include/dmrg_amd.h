@@ -59,7 +59,8 @@ typedef struct dmrg_options {
   double svd_cutoff;     /* absolute singular-value cutoff, 1e-14 (:1276)    */
   double svd_rel_cutoff; /* relative cutoff (sigma_i < rel*sigma_0); 0 = off */
   int precision;         /* DMRG_PREC_*                                       */
-  int profile;           /* 1: record per-phase HIP-event timings             */
+  int profile;           /* 1: time every H_eff matvec with HIP events (stats
+                            heff_ms); 0 (default): no events, heff_ms = 0     */
   int device;            /* HIP device ordinal                                */
   int svd_max_sweeps;    /* Jacobi sweep cap of the SVD eigen-solver          */
   double svd_tol;        /* Jacobi stop: max |rho_ij| <= svd_tol * |rho|_F    */
