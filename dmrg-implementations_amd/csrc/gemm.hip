@@ -96,8 +96,27 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
     if (!KFB) { n = t % BN; k = t / BN + r * (256 / BN); }
     else      { k = t % BK; n = t / BK + r * (256 / BK); }
   };
-  // register stage `s` <- operand tiles at depth k0
+  // register stage `s` <- operand tiles at depth k0; a tile inside all three
+  // bounds loads without per-element checks (one branch for the whole stage)
+  const bool inner_mn = m0 + BM <= M && n0 + BN <= N;
   auto load = [&](int s, int k0) {
+    if (inner_mn && k0 + BK <= K) {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        int m, k;
+        idxA(r, m, k);
+        const T v = A[aa(m0 + m, k0 + k)];
+        ra[s][r] = CA ? conj_v(v) : v;
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        int n, k;
+        idxB(r, n, k);
+        const T v = B[ab(k0 + k, n0 + n)];
+        rb[s][r] = CB ? conj_v(v) : v;
+      }
+      return;
+    }
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
       int m, k;
