@@ -261,16 +261,15 @@ void launch(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* 
                      alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC, herm)
   // 32x32 tiles serve grids that cannot fill the chip with 64x64 ones; a deep
   // k-step and two register stages then hide the load latency that the low
-  // tile count per CU leaves exposed (real types; complex already issues four
-  // MFMAs per k-step).
+  // tile count per CU leaves exposed.
   // 64x64 tiles: two register stages as well (measured on the H_eff shapes:
-  // FP64 +12-15 % with BK=16, FP32 +5-26 % with BK=32); complex keeps one
-  // stage (four MFMAs per k-step and twice the registers per element).
+  // FP64 +12-15 % with BK=16, FP32 +5-26 % with BK=32).  Complex: BK=16 and
+  // two stages for both tiles (+4-13 % on the C3 shapes; four MFMAs per k-step).
   if (tl.bm == 32 && tl.bn == 32) {
-    if constexpr (is_cplx<T>::value) DMRG_GEMM_LAUNCH(32, 32, 16, 1);
+    if constexpr (is_cplx<T>::value) DMRG_GEMM_LAUNCH(32, 32, 16, 2);
     else DMRG_GEMM_LAUNCH(32, 32, 32, 2);
   } else {
-    if constexpr (is_cplx<T>::value) DMRG_GEMM_LAUNCH(64, 64, 16, 1);
+    if constexpr (is_cplx<T>::value) DMRG_GEMM_LAUNCH(64, 64, 16, 2);
     else if constexpr (std::is_same<T, float>::value) DMRG_GEMM_LAUNCH(64, 64, 32, 2);
     else DMRG_GEMM_LAUNCH(64, 64, 16, 2);
   }
@@ -332,8 +331,10 @@ void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T
 GemmTile gemm_pick_tile(int M, int N, int K, bool complex_) {
   (void)K;
   const long long t64 = (long long)cdiv(M, 64) * cdiv(N, 64);
-  // 64x64 once there are about four tiles per CU (256 CUs)
-  if (t64 < (complex_ ? 256 : 1024)) return {32, 32};
+  // 64x64 once there are about four tiles per CU (256 CUs); complex too
+  // (measured at 1280^3: 32x32 with two stages 53.8 TF/s, 64x64 48.3)
+  (void)complex_;
+  if (t64 < 1024) return {32, 32};
   return {64, 64};
 }
 
