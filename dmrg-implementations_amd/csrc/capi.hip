@@ -466,7 +466,8 @@ int dmrg_op_svd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, 
       using T = decltype(tag);
       auto& x = ctx_of<T>(c);
       hipStream_t st = (hipStream_t)stream;
-      x.need_svd(scale_left ? n : m);
+      // the staging buffers hold products up to max(m, n) on a side (U_k^H M is k x n)
+      x.need_svd(std::max(m, n));
       SvdResult r = svd_split<T>(st, x.svd, (const T*)M, m, n, m, k, cutoff, rel_cutoff,
                                  scale_left != 0, (T*)left, m, (T*)right, sigma, SvdParams{});
       DMRG_HIP(hipStreamSynchronize(st));

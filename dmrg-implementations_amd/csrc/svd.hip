@@ -1266,20 +1266,31 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
   const int Np = round_up(N, JP);
   DMRG_REQUIRE(Np <= w.cap, "svd: matrix exceeds workspace");
   const int k = std::min(std::min(m, n), kmax);
-  // density matrix on the side of the orthonormal factor
-  if (!scale_left)
-    gemm_herm<T>(st, OP_N, OP_H, m, n, T(1), A, lda, A, lda, w.A[0], Np,
-                 gemm_pick_tile(m, m, n, is_cplx<T>::value));
-  else
-    gemm_herm<T>(st, OP_H, OP_N, n, m, T(1), A, lda, A, lda, w.A[0], Np,
-                 gemm_pick_tile(n, n, m, is_cplx<T>::value));
+  // density matrix on the side of the orthonormal factor; from the caller's
+  // orthonormal basis X (warm start, V = X) directly in that basis:
+  // X^H theta theta^H X = Y Y^H with Y = X^H theta ('R'), X^H theta^H theta X =
+  // Y^H Y with Y = theta X ('L'): one GEMM and one Hermitian product instead
+  // of three products
   if (warm) {
-    // start from the caller's orthonormal basis X: rho <- X^H rho X, V = X
     hipLaunchKernelGGL(k_set_basis<T>, dim3(grid_for((size_t)Np * Np)), dim3(256), 0, st, w.V[0],
                        warm, N, Np);
-    const GemmTile tl = gemm_pick_tile(N, N, N, is_cplx<T>::value);
-    gemm<T>(st, OP_N, OP_N, N, N, N, T(1), w.A[0], Np, w.V[0], Np, T(0), w.A[1], Np, tl);
-    gemm_herm<T>(st, OP_H, OP_N, N, N, T(1), w.V[0], Np, w.A[1], Np, w.A[0], Np, tl);
+    if (!scale_left) {  // N = m; Y (m x n, ld m) in w.Y
+      gemm<T>(st, OP_H, OP_N, m, n, m, T(1), w.V[0], Np, A, lda, T(0), w.Y, m,
+              gemm_pick_tile(m, n, m, is_cplx<T>::value));
+      gemm_herm<T>(st, OP_N, OP_H, m, n, T(1), w.Y, m, w.Y, m, w.A[0], Np,
+                   gemm_pick_tile(m, m, n, is_cplx<T>::value));
+    } else {  // N = n; Y (m x n, ld m) in w.Y
+      gemm<T>(st, OP_N, OP_N, m, n, n, T(1), A, lda, w.V[0], Np, T(0), w.Y, m,
+              gemm_pick_tile(m, n, n, is_cplx<T>::value));
+      gemm_herm<T>(st, OP_H, OP_N, n, m, T(1), w.Y, m, w.Y, m, w.A[0], Np,
+                   gemm_pick_tile(n, n, m, is_cplx<T>::value));
+    }
+  } else if (!scale_left) {
+    gemm_herm<T>(st, OP_N, OP_H, m, n, T(1), A, lda, A, lda, w.A[0], Np,
+                 gemm_pick_tile(m, m, n, is_cplx<T>::value));
+  } else {
+    gemm_herm<T>(st, OP_H, OP_N, n, m, T(1), A, lda, A, lda, w.A[0], Np,
+                 gemm_pick_tile(n, n, m, is_cplx<T>::value));
   }
   SvdResult res;
   res.N = N;

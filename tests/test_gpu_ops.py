@@ -129,11 +129,13 @@ def test_env_updates(dmrg_amd, ops, cplx, ci, co, d, D):
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("scale_left", [False, True])
 @pytest.mark.parametrize("m,n", [(4, 4), (1, 8), (8, 1), (30, 17), (17, 30), (64, 64), (130, 96),
-                                 (256, 256)])
+                                 (256, 256), (8, 700), (700, 8)])
 def test_svd_split(dmrg_amd, ops, cplx, scale_left, m, n):
     """Full-rank split against LAPACK: singular values to 1e-12 sigma_max, the
     orthonormal factor orthonormal to 1e-12, exact reconstruction, and the
-    product factor's Gram diagonal (S^2) to 1e-12 sigma_max^2."""
+    product factor's Gram diagonal (S^2) to 1e-12 sigma_max^2.  The very wide
+    and tall shapes stage a k x max(m, n) product: the workspace is sized by
+    max(m, n), not by the side of the density matrix."""
     rng = np.random.default_rng(m * 7 + n + (1000 if scale_left else 0))
     M = rand(rng, (m, n), cplx)
     k = min(m, n)
