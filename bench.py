@@ -290,8 +290,8 @@ def main():
                          "copy kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; the DMRG sweep of a single chain does not "
-              "shard across GPUs here (see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: measures one GPU only; the sweep of a single chain is sequential over its "
+              "bonds and is not sharded across GPUs.  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import dmrg_amd
     wl = WORKLOADS[a.workload]
