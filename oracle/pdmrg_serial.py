@@ -16,11 +16,13 @@ environments/update.py:16-153.  scipy's ARPACK ``eigsh`` is the eigensolver,
 as in the reference (scipy is a third-party dependency; uv.lock pins
 scipy 1.13.1, this image has 1.15.3).
 
-Pinning (tests/test_oracle_pdmrg.py): the H_eff and environment einsum
-identities of test_numerics.py:112-126 / :162-192 written in these
-conventions, the product-state energy (L-1)/4 (:212-227), and the serial
-sweep reaching exact-diagonalisation energies (the reference test asserts
-1e-6 against quimb DMRG2, :229-287).
+Pinning (tests/test_oracle_dmrg.py): apply_heff and update_left_env /
+update_right_env against the H_eff and environment einsums of
+test_numerics.py:112-126 / :162-192 written in these conventions
+(test_apply_heff2_matches_reference_einsum, test_update_envs_match_reference_einsums),
+and the serial sweep reaching the exact-diagonalisation energies of the
+oracle's own DMRG (test_pdmrg_serial_matches_oracle; the reference test
+asserts 1e-6 against quimb DMRG2, :229-287).
 """
 import numpy as np
 from scipy.sparse.linalg import LinearOperator, eigsh
