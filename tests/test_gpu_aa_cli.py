@@ -54,8 +54,3 @@ def test_cli_quiet_keeps_result_lines():
     assert m and abs(float(m.group(1)) + 3.374932598688) < 1e-8, out
     assert "Total wall time:" in out and "env_build_sec:" in out
     assert "PASS" not in out
-
-
-def test_cli_unknown_flag_fails():
-    rc, out, err = run_cli(["8", "32", "2", "--no-such-flag"])
-    assert rc == 1 and "Unknown flag" in err
