@@ -77,7 +77,7 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
   T* Zn = Z1;
   // dev[it*kPolarParts + blk]: workgroup maxima of |Z_it^H Z_it - I|; the host
   // reduces them after the first step and after steps 9, 11, 13, ... (from
-  // l = 1e-3 the scaled iteration needs about 12; steps past convergence keep
+  // l = 3e-4 the scaled iteration needs about 13; steps past convergence keep
   // Z unitary to rounding, the iteration being stable at its fixed point, and
   // by then a = 1: the bound l reaches 1 no later than the singular values).
   static thread_local std::vector<double> hp;

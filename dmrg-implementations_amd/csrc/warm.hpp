@@ -33,7 +33,7 @@ void overlap_right_step(hipStream_t st, const T* Tin, int r, int c, const T* Bol
 
 constexpr int kPolarMaxIter = 40;
 constexpr int kPolarParts = 256;  // workgroups of the deviation reduction
-constexpr double kPolarL0 = 1e-3;  // initial lower bound of the smallest singular value
+constexpr double kPolarL0 = 3e-4;  // initial lower bound of the smallest singular value
 
 // Unitary polar factor of the n x n matrix Tin by the scaled Newton-Schulz
 // iteration Z <- (a/2) Z (3I - a^2 Z^H Z) of Chen & Chow (2014), which maps
