@@ -156,6 +156,11 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
           std::lock_guard<std::mutex> lk(mu);
           if (!err_code) { err_code = ST_HIP; err_msg = "chain " + std::to_string(i) + ": " + e.what(); }
           stop = true;
+        } catch (...) {
+          // an exception escaping a worker thread would end the process
+          std::lock_guard<std::mutex> lk(mu);
+          if (!err_code) { err_code = ST_HIP; err_msg = "chain " + std::to_string(i) + ": unknown error"; }
+          stop = true;
         }
       }
     };
@@ -211,7 +216,7 @@ int dmrg_get_energy(dmrg_handle h, double* energy) {
 }
 
 int dmrg_get_sweep_energies(dmrg_handle h, double* out, int cap, int* n) {
-  if (!h || !n) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
+  if (!h || !n || (cap > 0 && !out)) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
     h->with([&](auto& e) {
       *n = (int)e.sweep_E.size();
@@ -357,6 +362,9 @@ void by_dtype(int dtype, F&& f) {
   else throw Error(ST_BAD_ARG, "bad dtype");
 }
 
+// Every op runs on its context's device, whichever thread calls it.
+void use(const dmrg_ctx* c) { DMRG_HIP(hipSetDevice(c->device)); }
+
 template <class T>
 std::vector<T> d2h(const void* p, size_t n) {
   std::vector<T> h(n);
@@ -390,6 +398,7 @@ int dmrg_op_heff2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, in
                   void* out) {
   if (!c || !L || !R || !W1 || !W2 || !theta || !out) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(cL >= 1 && cR >= 1 && d >= 1 && D >= 1, "bad shape");
     by_dtype(dtype, [&](auto tag) {
       using T = decltype(tag);
@@ -411,6 +420,7 @@ int dmrg_op_heff1(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, in
                   const void* L, const void* R, const void* W, const void* theta, void* out) {
   if (!c || !L || !R || !W || !theta || !out) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(cL >= 1 && cR >= 1 && d >= 1 && D >= 1, "bad shape");
     by_dtype(dtype, [&](auto tag) {
       using T = decltype(tag);
@@ -426,6 +436,7 @@ int dmrg_op_env_left(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co,
                      const void* L, const void* A, const void* W, void* Lnew) {
   if (!c || !L || !A || !W || !Lnew) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(ci >= 1 && co >= 1 && d >= 1 && D >= 1, "bad shape");
     by_dtype(dtype, [&](auto tag) {
       using T = decltype(tag);
@@ -440,6 +451,7 @@ int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co
                       const void* R, const void* A, const void* W, void* Rnew) {
   if (!c || !R || !A || !W || !Rnew) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(ci >= 1 && co >= 1 && d >= 1 && D >= 1, "bad shape");
     by_dtype(dtype, [&](auto tag) {
       using T = decltype(tag);
@@ -459,8 +471,9 @@ int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co
 int dmrg_op_svd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, int k,
                       double cutoff, double rel_cutoff, int scale_left, const void* M, void* left,
                       void* right, double* sigma, int* nk_out) {
-  if (!c || !M || !nk_out) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
+  if (!c || !M || !left || !right || !nk_out) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(m >= 1 && n >= 1 && k >= 1 && k <= std::min(m, n), "bad shape");
     by_dtype(dtype, [&](auto tag) {
       using T = decltype(tag);
@@ -480,6 +493,7 @@ int dmrg_op_heig(dmrg_ctx_handle c, void* stream, int dtype, int N, const void* 
                  void* evecs, int* sweeps_out) {
   if (!c || !H || !evals || !evecs) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(N >= 1 && N <= 2048, "bad shape");
     by_dtype(dtype, [&](auto tag) {
       using T = decltype(tag);
@@ -497,6 +511,7 @@ int dmrg_op_tridiag_lowest(dmrg_ctx_handle c, int k, const double* alpha, const 
                            double* lam, double* y) {
   if (!c || !alpha || !lam || !y || (k > 1 && !beta)) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(k >= 1 && k <= 128, "k must be in [1, 128]");
     auto& x = c->r;
     x.need_lz(128, 1);
@@ -516,6 +531,7 @@ int dmrg_op_lanczos2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR,
                      int krylov_max, int site, double* energy, int* niter) {
   if (!c || !L || !R || !W1 || !W2 || !theta || !energy) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(cL >= 1 && cR >= 1 && d >= 1 && D >= 1, "bad shape");
     DMRG_REQUIRE(krylov_max >= 1 && krylov_max <= 128, "krylov_max must be in [1, 128]");
     by_dtype(dtype, [&](auto tag) {
@@ -563,6 +579,7 @@ int dmrg_op_gemm(dmrg_ctx_handle c, void* stream, int dtype, int opA, int opB, i
                  const void* beta, void* Cm, int ldc) {
   if (!c || !alpha || !beta || !A || !B || !Cm) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(M >= 1 && N >= 1 && K >= 1, "bad shape");
     DMRG_REQUIRE(ldc >= M, "ldc < M");
     DMRG_REQUIRE(lda >= (op_trans(opA) ? K : M) && ldb >= (op_trans(opB) ? N : K), "bad ld");
@@ -579,6 +596,7 @@ int dmrg_op_gemm_f32(dmrg_ctx_handle c, void* stream, int prec, int opA, int opB
                      int K, const float* A, int lda, const float* B, int ldb, float* Cm, int ldc) {
   if (!c || !A || !B || !Cm) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] {
+    use(c);
     DMRG_REQUIRE(M >= 1 && N >= 1 && K >= 1, "bad shape");
     DMRG_REQUIRE((opA == OP_N || opA == OP_T) && (opB == OP_N || opB == OP_T), "op must be N or T");
     DMRG_REQUIRE(ldc >= M, "ldc < M");

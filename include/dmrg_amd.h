@@ -155,7 +155,8 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
 
 /* Replaces get_energy() (dmrg2_gpu.h:40). */
 int dmrg_get_energy(dmrg_handle h, double* energy);
-/* Energies at the end of each completed sweep (returned count in *n). */
+/* Energies at the end of each completed sweep: the count in *n, the first
+ * min(cap, *n) of them in out (which may be NULL when cap <= 0). */
 int dmrg_get_sweep_energies(dmrg_handle h, double* out, int cap, int* n);
 /* Replaces chi_L / chi_R (dmrg2_gpu.h:42-43): dims[0..L] of all bonds. */
 int dmrg_get_bond_dims(dmrg_handle h, int* dims);
@@ -181,7 +182,8 @@ const char* dmrg_last_error(void);
 /* ------------------------------------------------------------------------
  * Operator-level entry points on device buffers (for parity tests and the
  * roofline bench).  Each replaces one library-call sequence of the reference
- * and runs on the given HIP stream (0 = default stream); `ctx` owns scratch.
+ * and runs on the given HIP stream (0 = default stream); `ctx` owns scratch
+ * and names the device, which each call makes current on its thread.
  * ---------------------------------------------------------------------- */
 typedef struct dmrg_ctx* dmrg_ctx_handle;
 int dmrg_ctx_create(int device, dmrg_ctx_handle* out);

@@ -24,3 +24,26 @@ def test_errors_without_device_are_reported(dmrg_amd):
         assert e.code == dmrg_amd.ERR_HIP
     else:
         raise AssertionError("engine created without a device")
+
+
+def test_null_outputs_rejected_before_any_device_call(dmrg_amd):
+    """Output pointers that an entry point would write through are checked on
+    the host first; the handle below is only a non-NULL stand-in, never
+    dereferenced on these paths."""
+    import ctypes as C
+
+    lib = dmrg_amd.lib()
+    stand_in = C.create_string_buffer(64)
+    h = C.cast(stand_in, C.c_void_p)
+    n = C.c_int(-1)
+    assert lib.dmrg_get_sweep_energies(h, None, 4, C.byref(n)) == dmrg_amd.ERR_BAD_ARG
+    assert n.value == -1
+    nk = C.c_int(-1)
+    data = C.create_string_buffer(64)
+    buf = C.cast(data, C.c_void_p)
+    for left, right in ((None, buf), (buf, None)):
+        rc = lib.dmrg_op_svd_split(h, None, dmrg_amd.DTYPE_F64, 4, 4, 2, 1e-14, 0.0, 0, buf,
+                                   left, right, None, C.byref(nk))
+        assert rc == dmrg_amd.ERR_BAD_ARG
+        assert b"NULL" in lib.dmrg_last_error()
+    assert nk.value == -1
