@@ -152,6 +152,30 @@ def device_count():
     return n.value
 
 
+def dtype_code(dtype):
+    """DTYPE_F64 or DTYPE_C128 for "f64"/"float64"/float/np.float64 or
+    "c128"/"complex128"/complex/np.complex128 (or the codes themselves);
+    any other type raises ValueError rather than running in another one."""
+    if isinstance(dtype, (int, np.integer)) and not isinstance(dtype, bool):
+        if int(dtype) in (DTYPE_F64, DTYPE_C128):
+            return int(dtype)
+    elif isinstance(dtype, str):
+        if dtype in ("f64", "float64"):
+            return DTYPE_F64
+        if dtype in ("c128", "complex128"):
+            return DTYPE_C128
+    elif dtype is not None and not isinstance(dtype, bool):  # np.dtype(None) is float64
+        try:
+            nd = np.dtype(dtype)
+        except TypeError:
+            nd = None
+        if nd == np.float64:
+            return DTYPE_F64
+        if nd == np.complex128:
+            return DTYPE_C128
+    raise ValueError("dtype must be f64 or c128, got %r" % (dtype,))
+
+
 def _np_dtype(dtype):
     return np.float64 if dtype == DTYPE_F64 else np.complex128
 
@@ -164,7 +188,7 @@ class DMRG:
     """Handle on one engine (two-site or single-site, FP64 or complex128)."""
 
     def __init__(self, L, d, chi_max, D, tol=1e-10, dtype="f64", variant=TWO_SITE, **opts):
-        self.dtype = DTYPE_C128 if dtype in ("c128", "complex128", DTYPE_C128, complex) else DTYPE_F64
+        self.dtype = dtype_code(dtype)
         self.L, self.d, self.chi_max, self.D = L, d, chi_max, D
         self.opts = default_options(**opts)
         h = C.c_void_p()

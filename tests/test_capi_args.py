@@ -1,0 +1,34 @@
+"""Argument checks of the engine boundary that run before any device call, so
+they hold on any machine: the reference's constructor takes the same
+(L, d, chi_max, D) arguments (gpu-rocm/dmrg2-gpu/src/dmrg2_gpu.h:28-31)."""
+import numpy as np
+import pytest
+
+
+def test_dtype_spellings(dmrg_amd):
+    for f in ("f64", "float64", float, np.float64, dmrg_amd.DTYPE_F64):
+        assert dmrg_amd.dtype_code(f) == dmrg_amd.DTYPE_F64
+    for c in ("c128", "complex128", complex, np.complex128, dmrg_amd.DTYPE_C128):
+        assert dmrg_amd.dtype_code(c) == dmrg_amd.DTYPE_C128
+    for bad in ("f32", "bf16", np.float32, np.complex64, 2, True, None):
+        with pytest.raises(ValueError):
+            dmrg_amd.dtype_code(bad)
+
+
+@pytest.mark.parametrize("args, kw, msg", [
+    ((1, 2, 8, 5), {}, b"L must be"),
+    ((4, 0, 8, 5), {}, b"physical dimension"),
+    ((4, 2, 0, 5), {}, b"chi_max"),
+    ((4, 2, 8, 0), {}, b"chi_max"),
+    ((4, 2, 8, 5), {"variant": 3}, b"variant"),
+    ((4, 2, 8, 5), {"krylov_max": 0}, b"krylov_max"),
+    ((4, 2, 8, 5), {"krylov_max": 129}, b"krylov_max"),
+    ((4, 2, 1025, 5), {}, b"chi_max * d"),
+    ((4, 2, 8, 5), {"dtype": "c128", "precision": 1}, b"FP32 matvec"),
+    ((4, 2, 8, 5), {"variant": 1, "precision": 2}, b"FP32 matvec"),
+])
+def test_create_rejects_bad_arguments(dmrg_amd, args, kw, msg):
+    with pytest.raises(dmrg_amd.DMRGError) as ei:
+        dmrg_amd.DMRG(*args, **kw)
+    assert ei.value.code == dmrg_amd.ERR_BAD_ARG
+    assert msg.decode() in str(ei.value)
