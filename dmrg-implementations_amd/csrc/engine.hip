@@ -11,6 +11,12 @@
 namespace dmrg {
 
 namespace {
+template <class F>
+struct OnExit {
+  F f;
+  ~OnExit() { f(); }
+};
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -110,6 +116,7 @@ Engine<T>::Engine(int L_, int d_, int chi_, int D_, double tol_, int variant_, c
     double ex = std::pow((double)d, (double)e);
     bond[i] = ex > chi ? chi : (int)ex;
   }
+  sync_shapes();
   try {
     alloc_all();
   } catch (...) {
@@ -273,6 +280,7 @@ void Engine<T>::init_random(uint64_t seed, double scale) {
     off += n;
   }
   DMRG_HIP(hipStreamSynchronize(s_main));
+  sync_shapes();
 }
 
 template <class T>
@@ -281,6 +289,7 @@ void Engine<T>::init_product() {
   std::vector<T> h(d, T(0));
   h[0] = T(1);
   for (int i = 0; i < L; ++i) DMRG_HIP(hipMemcpy(mps[i], h.data(), d * sizeof(T), hipMemcpyHostToDevice));
+  sync_shapes();
 }
 
 template <class T>
@@ -292,6 +301,24 @@ void Engine<T>::init_neel() {
     for (int a = 0; a < std::min(cL, cR); ++a) h[a + (size_t)s * cL + (size_t)a * cL * d] = T(1);
     DMRG_HIP(hipMemcpy(mps[i], h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   }
+  sync_shapes();
+}
+
+template <class T>
+void Engine<T>::sync_shapes() {
+  tl_.assign(bond.begin(), bond.end() - 1);
+  tr_.assign(bond.begin() + 1, bond.end());
+}
+
+template <class T>
+void Engine<T>::check_shapes(const char* who) const {
+  for (int i = 0; i < L; ++i)
+    if (tl_[i] != bond[i] || tr_[i] != bond[i + 1])
+      throw Error(ST_STATE, std::string(who) + ": site " + std::to_string(i) + " holds a " +
+                                std::to_string(tl_[i]) + " x d x " + std::to_string(tr_[i]) +
+                                " tensor but its bonds are " + std::to_string(bond[i]) + " and " +
+                                std::to_string(bond[i + 1]) +
+                                " (set_mps shapes disagree between neighbouring sites)");
 }
 
 template <class T>
@@ -305,6 +332,8 @@ void Engine<T>::set_mps(int site, const T* A, int cL, int cR) {
     if (!finite_v(A[k])) throw Error(ST_NONFINITE, "set_mps: non-finite entry");
   bond[site] = cL;
   bond[site + 1] = cR;
+  tl_[site] = cL;
+  tr_[site] = cR;
   DMRG_HIP(hipMemcpy(mps[site], A, n * sizeof(T), hipMemcpyHostToDevice));
 }
 
@@ -687,6 +716,10 @@ template <class T>
 double Engine<T>::run(int n_sweeps, bool quiet) {
   if (!mpo_set) throw Error(ST_STATE, "run: set_mpo has not been called");
   DMRG_REQUIRE(n_sweeps >= 0, "run: negative sweep count");
+  check_shapes("run");
+  // the sweeps keep every tensor consistent with bond[], however they end
+  auto resync = [this] { sync_shapes(); };
+  OnExit<decltype(resync)> resync_at_exit{resync};
   DMRG_HIP(hipSetDevice(opt.device));  // the calling thread may not be the creating one
   double te = now_s();
   build_envs();
@@ -731,12 +764,19 @@ double Engine<T>::run(int n_sweeps, bool quiet) {
 template <class T>
 void Engine<T>::expectation(double* e, double* nrm2) {
   if (!mpo_set) throw Error(ST_STATE, "expectation: set_mpo has not been called");
+  check_shapes("expectation");
   DMRG_HIP(hipStreamSynchronize(s_main));
   // Use Lenv[1..] as scratch is not allowed (they hold state); ping-pong in
   // the spare MPS-sized buffers instead: env (chi, D, chi) fits in work/theta
   // only when D <= d*d (two-site) -- use dedicated temporaries.
   const size_t esz = (size_t)chi * D * chi;
   T *ea = nullptr, *eb = nullptr, *wid = nullptr;
+  auto release = [&] {
+    if (ea) (void)hipFree(ea);
+    if (eb) (void)hipFree(eb);
+    if (wid) (void)hipFree(wid);
+  };
+  OnExit<decltype(release)> release_at_exit{release};
   DMRG_HIP(hipMalloc(&ea, esz * sizeof(T)));
   DMRG_HIP(hipMalloc(&eb, esz * sizeof(T)));
   DMRG_HIP(hipMalloc(&wid, (size_t)d * d * sizeof(T)));
@@ -755,9 +795,6 @@ void Engine<T>::expectation(double* e, double* nrm2) {
     DMRG_HIP(hipMemcpyAsync(&hv[pass], ea + (Dp - 1), sizeof(T), hipMemcpyDeviceToHost, s_main));
     DMRG_HIP(hipStreamSynchronize(s_main));
   }
-  (void)hipFree(ea);
-  (void)hipFree(eb);
-  (void)hipFree(wid);
   *nrm2 = real(hv[1]);
   *e = real(hv[0]) / real(hv[1]);
 }

@@ -108,6 +108,11 @@ class Engine {
   Stats stats;
 
  private:
+  // Shape (cL, cR) of the tensor each site holds.  set_mps changes one site's
+  // bonds, so run and expectation check every site against bond[] first.
+  std::vector<int> tl_, tr_;
+  void sync_shapes();
+  void check_shapes(const char* who) const;
   void alloc_all();
   void free_all();
   void build_envs();

@@ -274,3 +274,35 @@ def test_frustrated_chain_and_ladder(dmrg_amd, model, L, args, E_ref):
     assert abs(E - E_ed) < 1e-9
     if E_ref is not None:
         assert abs(E - E_ref) < 1e-8
+
+
+def test_set_mps_shapes_checked_between_neighbours(dmrg_amd):
+    """Site tensors set one by one must agree on their shared bonds: a site
+    whose left bond disagrees with its neighbour's right bond makes run and
+    expectation fail with STATE before any kernel; a consistent MPS with its
+    own bond dimensions gives the oracle's <H> and then the ED ground state."""
+    L, chi = 6, 8
+    Ws = models.heisenberg_mpo(L)
+    rng = np.random.default_rng(5)
+    shapes = [(1, 2), (2, 3), (3, 3), (3, 3), (3, 2), (2, 1)]
+    mps = [rng.standard_normal((a, 2, b)) for a, b in shapes]
+    eng = dmrg_amd.DMRG(L, 2, chi, 5, krylov_max=20)
+    eng.set_mpo(Ws)
+    eng.set_mps(mps)
+    E0, n0 = eng.expectation()
+    assert E0 == pytest.approx(od.expectation(mps, Ws) / od.norm2(mps), rel=0, abs=1e-10)
+    assert n0 == pytest.approx(od.norm2(mps), rel=1e-12)
+    # sites 0-2 only, site 2 with a right bond of 4: site 3 still holds 3 x 2 x 3
+    eng.set_mps(mps[:2] + [rng.standard_normal((3, 2, 4))])
+    for call in (eng.expectation, lambda: eng.run(2)):
+        with pytest.raises(dmrg_amd.DMRGError) as ei:
+            call()
+        assert ei.value.code == dmrg_amd.ERR_STATE
+        assert "site 3" in str(ei.value)
+    eng.set_mps(mps)
+    E1, _ = eng.expectation()
+    assert E1 == pytest.approx(E0, rel=0, abs=1e-12 * abs(E0))
+    E = eng.run(8)
+    E_ed = models.ground_energy(models.dense_heisenberg(L))
+    assert abs(E - E_ed) < 1e-9
+    eng.close()
