@@ -130,7 +130,9 @@ int dmrg_init_mps_product(dmrg_handle h);
 /* Replaces initialize_mps_neel() (dmrg2_gpu_impl.h:415-429). */
 int dmrg_init_mps_neel(dmrg_handle h);
 /* Set one site tensor (cL, d, cR); the bond dimensions of the neighbouring
- * bonds become cL and cR.  No reference counterpart (the reference only has
+ * bonds become cL and cR.  Sites may be set one at a time: dmrg_run and
+ * dmrg_expectation return DMRG_ERR_STATE while a site's tensor disagrees with
+ * its neighbour's on a shared bond.  No reference counterpart (the reference only has
  * the initialize_mps_* generators); used to feed identical states to the
  * engine and to a CPU path, and to load the reference's .bin/.npz files. */
 int dmrg_set_mps(dmrg_handle h, int site, const void* A, int cL, int cR);
