@@ -245,6 +245,7 @@ void Engine<T>::free_all() {
 template <class T>
 void Engine<T>::set_mpo(const T* const* W) {
   DMRG_REQUIRE(W != nullptr, "set_mpo: null");
+  mpo_set = false;  // until every site is in: a rejected call leaves no half-set MPO usable
   const size_t wsz = (size_t)D * d * d * D;
   std::vector<T> wr(wsz);
   for (int i = 0; i < L; ++i) {
