@@ -215,10 +215,13 @@ class DMRG:
 
     # --- state setup -------------------------------------------------------
     def set_mpo(self, Ws):
-        assert len(Ws) == self.L
+        if len(Ws) != self.L:
+            raise ValueError("set_mpo: %d tensors for %d sites" % (len(Ws), self.L))
+        for i, W in enumerate(Ws):
+            if np.shape(W) != (self.D, self.d, self.d, self.D):
+                raise ValueError("set_mpo: site %d has shape %s, expected %s"
+                                 % (i, np.shape(W), (self.D, self.d, self.d, self.D)))
         flats = [_flat(W, self.dtype) for W in Ws]
-        for f in flats:
-            assert f.size == self.D * self.d * self.d * self.D, "MPO tensor size"
         ptrs = (C.c_void_p * self.L)(*[f.ctypes.data for f in flats])
         _check(lib().dmrg_set_mpo(self._h, ptrs))
 
@@ -232,6 +235,13 @@ class DMRG:
         _check(lib().dmrg_init_mps_neel(self._h))
 
     def set_mps(self, mps):
+        """Set sites 0 .. len(mps)-1; each A has shape (cL, d, cR)."""
+        if len(mps) > self.L:
+            raise ValueError("set_mps: %d tensors for %d sites" % (len(mps), self.L))
+        for i, A in enumerate(mps):
+            if np.ndim(A) != 3 or np.shape(A)[1] != self.d:
+                raise ValueError("set_mps: site %d has shape %s, expected (cL, %d, cR)"
+                                 % (i, np.shape(A), self.d))
         for i, A in enumerate(mps):
             f = _flat(A, self.dtype)
             _check(lib().dmrg_set_mps(self._h, i, f.ctypes.data, A.shape[0], A.shape[2]))

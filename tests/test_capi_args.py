@@ -32,3 +32,26 @@ def test_create_rejects_bad_arguments(dmrg_amd, args, kw, msg):
         dmrg_amd.DMRG(*args, **kw)
     assert ei.value.code == dmrg_amd.ERR_BAD_ARG
     assert msg.decode() in str(ei.value)
+
+
+def _unbound_engine(dmrg_amd, L=4, d=2, D=5):
+    """A DMRG object with no library handle: only the binding's own checks,
+    which run before any library call, can be exercised on it."""
+    e = dmrg_amd.DMRG.__new__(dmrg_amd.DMRG)
+    e.L, e.d, e.D, e.dtype, e._h = L, d, D, dmrg_amd.DTYPE_F64, None
+    return e
+
+
+def test_binding_checks_tensor_shapes(dmrg_amd):
+    e = _unbound_engine(dmrg_amd)
+    W = np.zeros((5, 2, 2, 5))
+    with pytest.raises(ValueError, match="4 sites"):
+        e.set_mpo([W] * 3)
+    with pytest.raises(ValueError, match="site 2"):
+        e.set_mpo([W, W, np.zeros((5, 3, 3, 5)), W])
+    with pytest.raises(ValueError, match="site 1"):  # d = 1: the library would read past it
+        e.set_mps([np.zeros((1, 2, 2)), np.zeros((2, 1, 2))])
+    with pytest.raises(ValueError, match="site 0"):
+        e.set_mps([np.zeros((2, 2))])
+    with pytest.raises(ValueError, match="5 tensors"):
+        e.set_mps([np.zeros((1, 2, 1))] * 5)

@@ -306,3 +306,26 @@ def test_set_mps_shapes_checked_between_neighbours(dmrg_amd):
     E_ed = models.ground_energy(models.dense_heisenberg(L))
     assert abs(E - E_ed) < 1e-9
     eng.close()
+
+
+def test_rejected_set_mpo_leaves_no_mpo(dmrg_amd):
+    """A non-finite entry in the last site's MPO is refused (NONFINITE); the
+    sites copied before it do not count as a set MPO, so run reports STATE
+    until a complete MPO is given."""
+    L = 6
+    Ws = models.heisenberg_mpo(L)
+    eng = dmrg_amd.DMRG(L, 2, 8, 5, krylov_max=20)
+    eng.set_mpo(Ws)
+    eng.initialize_mps_random(seed=3, scale=0.1)
+    bad = [W.copy() for W in Ws]
+    bad[-1][0, 0, 0, 0] = np.nan
+    with pytest.raises(dmrg_amd.DMRGError) as ei:
+        eng.set_mpo(bad)
+    assert ei.value.code == dmrg_amd.ERR_NONFINITE
+    with pytest.raises(dmrg_amd.DMRGError) as ei:
+        eng.run(1)
+    assert ei.value.code == dmrg_amd.ERR_STATE
+    eng.set_mpo(Ws)
+    E = eng.run(8)
+    assert abs(E - models.ground_energy(models.dense_heisenberg(L))) < 1e-9
+    eng.close()
