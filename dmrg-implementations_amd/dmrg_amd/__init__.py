@@ -367,39 +367,66 @@ class Ops:
 
     @staticmethod
     def _dt(arr):
-        return DTYPE_C128 if np.dtype(arr.dtype) == np.complex128 else DTYPE_F64
+        return dtype_code(arr.dtype)
+
+    @staticmethod
+    def _expect(dtype, **arrays):
+        """Every named device array must have the given shape and dtype: the
+        library reads and writes exactly those extents, so a mismatch is
+        refused here rather than becoming an out-of-bounds device access."""
+        for name, (arr, shape) in arrays.items():
+            if tuple(arr.shape) != tuple(shape):
+                raise ValueError("%s: shape %s, expected %s" % (name, tuple(arr.shape), tuple(shape)))
+            if np.dtype(arr.dtype) != np.dtype(dtype):
+                raise ValueError("%s: dtype %s, expected %s" % (name, arr.dtype, np.dtype(dtype)))
+
+    def _two_site(self, L, R, W1, W2, theta, out=None):
+        cL, D = L.shape[0], L.shape[1]
+        cR, d = R.shape[0], theta.shape[1]
+        arrays = dict(L=(L, (cL, D, cL)), R=(R, (cR, D, cR)), W1=(W1, (D, d, d, D)),
+                      W2=(W2, (D, d, d, D)), theta=(theta, (cL, d, d, cR)))
+        if out is not None:
+            arrays["out"] = (out, (cL, d, d, cR))
+        self._expect(theta.dtype, **arrays)
+        return cL, cR, d, D
 
     def heff2(self, L, R, W1, W2, theta, out, stream=None):
-        cL, D, _ = L.shape
-        cR = R.shape[0]
-        d = theta.shape[1]
+        cL, cR, d, D = self._two_site(L, R, W1, W2, theta, out)
         _check(lib().dmrg_op_heff2(self._h, stream, self._dt(theta), cL, cR, d, D, L.ptr, R.ptr,
                                    W1.ptr, W2.ptr, theta.ptr, out.ptr))
 
     def heff1(self, L, R, W, theta, out, stream=None):
-        cL, D, _ = L.shape
-        cR = R.shape[0]
-        d = theta.shape[1]
+        cL, D = L.shape[0], L.shape[1]
+        cR, d = R.shape[0], theta.shape[1]
+        self._expect(theta.dtype, L=(L, (cL, D, cL)), R=(R, (cR, D, cR)), W=(W, (D, d, d, D)),
+                     theta=(theta, (cL, d, cR)), out=(out, (cL, d, cR)))
         _check(lib().dmrg_op_heff1(self._h, stream, self._dt(theta), cL, cR, d, D, L.ptr, R.ptr,
                                    W.ptr, theta.ptr, out.ptr))
 
     def env_left(self, L, A, W, Lnew, stream=None):
         ci, d, co = A.shape
         D = W.shape[0]
+        self._expect(A.dtype, L=(L, (ci, D, ci)), A=(A, (ci, d, co)), W=(W, (D, d, d, D)),
+                     Lnew=(Lnew, (co, D, co)))
         _check(lib().dmrg_op_env_left(self._h, stream, self._dt(A), ci, co, d, D, L.ptr, A.ptr,
                                       W.ptr, Lnew.ptr))
 
     def env_right(self, R, A, W, Rnew, stream=None):
         co, d, ci = A.shape
         D = W.shape[0]
+        self._expect(A.dtype, R=(R, (ci, D, ci)), A=(A, (co, d, ci)), W=(W, (D, d, d, D)),
+                     Rnew=(Rnew, (co, D, co)))
         _check(lib().dmrg_op_env_right(self._h, stream, self._dt(A), ci, co, d, D, R.ptr, A.ptr,
                                        W.ptr, Rnew.ptr))
 
     def svd_split(self, M, k, cutoff=1e-14, rel_cutoff=0.0, scale_left=False):
         """Truncated split of a device matrix M (m x n); returns numpy
         (left, right, sigma, nk)."""
+        if len(M.shape) != 2:
+            raise ValueError("svd_split: M must be 2-D, got shape %s" % (M.shape,))
         m, n = M.shape
         dt = M.dtype
+        dtype_code(dt)
         left = DeviceArray((m, k), dt)
         right = DeviceArray((k, n), dt)
         sig = DeviceArray((k,), np.float64)
@@ -416,6 +443,8 @@ class Ops:
         """Hermitian eigen-decomposition of a device matrix H (N x N); returns
         numpy (evals descending, evecs, sweeps)."""
         N = H.shape[0]
+        self._expect(H.dtype, H=(H, (N, N)))
+        dtype_code(H.dtype)
         ev = DeviceArray((N,), np.float64)
         V = DeviceArray((N, N), H.dtype)
         sw = C.c_int(0)
@@ -429,6 +458,8 @@ class Ops:
         npd = _np_dtype(dt)
         M, N = Cm.shape
         K = A.shape[0] if opA in (1, 3) else A.shape[1]
+        self._expect(npd, A=(A, (K, M) if opA in (1, 3) else (M, K)),
+                     B=(B, (N, K) if opB in (1, 3) else (K, N)), C=(Cm, (M, N)))
         a = np.array([alpha], dtype=npd)
         b = np.array([beta], dtype=npd)
         _check(lib().dmrg_op_gemm(self._h, stream, dt, opA, opB, M, N, K, a.ctypes.data, A.ptr,
@@ -439,6 +470,8 @@ class Ops:
         or split-BF16 (prec)."""
         M, N = Cm.shape
         K = A.shape[0] if opA == 1 else A.shape[1]
+        self._expect(np.float32, A=(A, (K, M) if opA == 1 else (M, K)),
+                     B=(B, (N, K) if opB == 1 else (K, N)), C=(Cm, (M, N)))
         _check(lib().dmrg_op_gemm_f32(self._h, stream, prec, opA, opB, M, N, K, A.ptr, A.shape[0],
                                       B.ptr, B.shape[0], Cm.ptr, M))
 
@@ -454,9 +487,7 @@ class Ops:
         return lam.value, y
 
     def lanczos2(self, L, R, W1, W2, theta, krylov_max=100, site=0, stream=None):
-        cL, D, _ = L.shape
-        cR = R.shape[0]
-        d = theta.shape[1]
+        cL, cR, d, D = self._two_site(L, R, W1, W2, theta)
         e = C.c_double(0)
         nit = C.c_int(0)
         _check(lib().dmrg_op_lanczos2(self._h, stream, self._dt(theta), cL, cR, d, D, L.ptr, R.ptr,

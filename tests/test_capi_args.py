@@ -55,3 +55,43 @@ def test_binding_checks_tensor_shapes(dmrg_amd):
         e.set_mps([np.zeros((2, 2))])
     with pytest.raises(ValueError, match="5 tensors"):
         e.set_mps([np.zeros((1, 2, 1))] * 5)
+
+
+def _unbound_ops(dmrg_amd):
+    o = dmrg_amd.Ops.__new__(dmrg_amd.Ops)
+    o._h = None
+    return o
+
+
+def _arr(shape, dtype=np.float64):
+    """Shape and dtype of a device array, without an allocation."""
+    from types import SimpleNamespace
+    return SimpleNamespace(shape=tuple(shape), dtype=np.dtype(dtype), ptr=None)
+
+
+def test_op_wrappers_refuse_mismatched_device_arrays(dmrg_amd):
+    """Each op reads and writes exactly the extents its sizes imply; arrays
+    that disagree are refused before the library call."""
+    o = _unbound_ops(dmrg_amd)
+    cL, cR, d, D = 4, 3, 2, 5
+    L, R, W = _arr((cL, D, cL)), _arr((cR, D, cR)), _arr((D, d, d, D))
+    th, out = _arr((cL, d, d, cR)), _arr((cL, d, d, cR))
+    cases = [
+        (lambda: o.heff2(L, R, W, W, th, _arr((cL, d, d, cR + 1))), "out: shape"),
+        (lambda: o.heff2(L, _arr((cR, D - 1, cR)), W, W, th, out), "R: shape"),
+        (lambda: o.heff2(L, R, W, _arr((D, d, d, D), np.complex128), th, out), "W2: dtype"),
+        (lambda: o.lanczos2(L, R, _arr((D, 3, 3, D)), W, th), "W1: shape"),
+        (lambda: o.heff1(L, R, W, _arr((cL, d, cR)), _arr((cL, d, d, cR))), "out: shape"),
+        (lambda: o.env_left(L, _arr((cL, d, 6)), W, _arr((6, D, 5))), "Lnew: shape"),
+        (lambda: o.env_right(R, _arr((6, d, cR)), W, _arr((6, D, 6), np.complex128)), "Rnew: dtype"),
+        (lambda: o.gemm(0, 0, _arr((7, 5)), _arr((4, 9)), _arr((7, 9))), "B: shape"),
+        (lambda: o.gemm(1, 0, _arr((5, 7)), _arr((5, 9)), _arr((8, 9))), "A: shape"),
+        (lambda: o.gemm_f32(0, 1, _arr((7, 5), np.float32), _arr((9, 5), np.float32),
+                            _arr((7, 9))), "C: dtype"),
+        (lambda: o.heig(_arr((6, 5))), "H: shape"),
+        (lambda: o.svd_split(_arr((6,)), 2), "2-D"),
+        (lambda: o.svd_split(_arr((6, 4), np.float32), 2), "f64 or c128"),
+    ]
+    for call, msg in cases:
+        with pytest.raises(ValueError, match=msg):
+            call()
