@@ -289,6 +289,11 @@ def main():
                     help="before the runs, copy MIB MiB device-to-device 3 times (the known-size "
                          "copy kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
+    bad = [m for m in a.modes.split(",") if m not in PEAK]
+    if bad:
+        ap.error("--modes: unknown mode(s) %s; choose from %s" % (",".join(bad), ",".join(PEAK)))
+    if a.steps < 1 or a.warmup < 0:
+        ap.error("--steps must be >= 1 and --warmup >= 0")
     if a.gpus != 1:
         print("bench.py: measures one GPU only; the sweep of a single chain is sequential over its "
               "bonds and is not sharded across GPUs.  Run with --gpus 1.", file=sys.stderr)
