@@ -120,6 +120,13 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
   for (int i = 0; i < n; ++i)
     if (!hs[i]) return fail(DMRG_ERR_BAD_ARG, "dmrg_run_batch: NULL engine");
   return guard([&] {
+    {
+      // one engine listed twice would run on two threads at once
+      std::vector<dmrg_handle> sorted(hs, hs + n);
+      std::sort(sorted.begin(), sorted.end());
+      DMRG_REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(),
+                   "dmrg_run_batch: the same engine is listed twice");
+    }
     // longest first: estimated cost L * chi^3 * d^3 per sweep
     std::vector<int> order(n);
     std::vector<double> cost(n);

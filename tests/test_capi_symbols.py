@@ -47,3 +47,18 @@ def test_null_outputs_rejected_before_any_device_call(dmrg_amd):
         assert rc == dmrg_amd.ERR_BAD_ARG
         assert b"NULL" in lib.dmrg_last_error()
     assert nk.value == -1
+
+
+def test_batch_refuses_an_engine_listed_twice(dmrg_amd):
+    """Both entries would run on two threads at once; the check precedes any
+    use of the handles (stand-ins here, never dereferenced)."""
+    import ctypes as C
+
+    lib = dmrg_amd.lib()
+    a, b = C.create_string_buffer(64), C.create_string_buffer(64)
+    pa, pb = C.cast(a, C.c_void_p).value, C.cast(b, C.c_void_p).value
+    hs = (C.c_void_p * 3)(pa, pb, pa)
+    E = (C.c_double * 3)()
+    rc = lib.dmrg_run_batch(hs, 3, 1, 2, C.cast(E, C.c_void_p), None)
+    assert rc == dmrg_amd.ERR_BAD_ARG
+    assert b"twice" in lib.dmrg_last_error()
