@@ -44,20 +44,22 @@ def test_apply_heff2_matches_reference_einsum(cplx):
     assert np.abs(got2 - ref).max() < 1e-12 * np.abs(ref).max()
 
 
-def test_update_envs_match_reference_einsums():
-    """test_numerics.py:162-192."""
+@pytest.mark.parametrize("cplx", [False, True])
+def test_update_envs_match_reference_einsums(cplx):
+    """test_numerics.py:162-192 (real there; complex tensors here as well,
+    where the einsums' conjugate of the bra tensor matters)."""
     rng = np.random.default_rng(42)
     chi, chi2, D, d = 3, 4, 5, 2
-    Lp = _r(rng, (chi, D, chi))
-    A = _r(rng, (chi, d, chi2))
+    Lp = _r(rng, (chi, D, chi), cplx)
+    A = _r(rng, (chi, d, chi2), cplx)
     Wp = _r(rng, (D, D, d, d))
     refL = np.einsum("awc,atp,wmst,csq->pmq", Lp, A.conj(), Wp, A, optimize=True)
     W = Wp.transpose(0, 2, 3, 1)
     gotL = od.update_left_env(Lp.transpose(2, 1, 0), A, W)          # (ket, mpo, bra)
     assert np.abs(gotL.transpose(2, 1, 0) - refL).max() < 1e-12
     assert np.abs(ps.update_left_env(Lp, A, Wp) - refL).max() < 1e-12
-    Rp = _r(rng, (chi, D, chi))
-    B = _r(rng, (chi2, d, chi))
+    Rp = _r(rng, (chi, D, chi), cplx)
+    B = _r(rng, (chi2, d, chi), cplx)
     refR = np.einsum("awc,pta,mwst,qsc->pmq", Rp, B.conj(), Wp, B, optimize=True)
     gotR = od.update_right_env(Rp.transpose(2, 1, 0), B, W)
     assert np.abs(gotR.transpose(2, 1, 0) - refR).max() < 1e-12
@@ -155,6 +157,11 @@ def test_svd_split2_truncation_rule():
     assert abs(disc - np.sum(s[3:] ** 2)) < 1e-15
     rec = np.tensordot(Al, Ar, axes=([2], [0])).reshape(cL * d, d * cR, order="F")
     assert np.linalg.norm(rec - (U[:, :3] * s[:3]) @ V[:, :3].T) < 1e-12
+    # the cutoff is absolute: with sigma_0 = 1e-3 the first sigma below 1e-14
+    # is 1e-16 (index 5); a cutoff relative to sigma_0 would keep 6
+    th3 = (th * 1e-3).copy()
+    Al, Ar, S, disc = od.svd_split2(th3, 100, "R")
+    assert Al.shape[2] == 5
 
 
 @pytest.mark.parametrize("L,E", [(4, -1.616025403784), (8, -3.374932598688)])

@@ -1,0 +1,116 @@
+"""CPU: every pin of the oracle rejects a plausible mistake planted in the
+oracle.
+
+Each test plants one mistake in the oracle on the CPU (a transposed operand, a
+swapped index pair, a dropped conjugate, a dropped or sign-flipped term, a
+relative cutoff in place of the absolute one, an off-by-one in the random
+stream) by monkeypatching, then runs a pin from test_oracle_dmrg.py or
+test_oracle_models.py unchanged and requires it to fail.  The unmutated pins
+pass in those modules.
+"""
+import numpy as np
+import pytest
+
+import test_oracle_dmrg as dmrg_pins
+import test_oracle_models as model_pins
+from oracle import dmrg as od
+from oracle import models, prng
+
+
+def _caught(pin, *args):
+    """True when the pin fails (an assertion, or an error a mistake causes)."""
+    try:
+        pin(*args)
+    except (AssertionError, ValueError, np.linalg.LinAlgError):
+        return True
+    return False
+
+
+class _NoConj(np.ndarray):
+    """An array whose conj() is itself: plants a dropped complex conjugate."""
+
+    def conj(self):
+        return self
+
+
+def test_transposed_mpo_operand_in_heff(monkeypatch):
+    orig = od.apply_heff2
+    monkeypatch.setattr(od, "apply_heff2",
+                        lambda Lg, Rg, W1, W2, th: orig(Lg, Rg, W1.transpose(0, 2, 1, 3), W2, th))
+    assert _caught(dmrg_pins.test_apply_heff2_matches_reference_einsum, True)
+
+
+def test_ket_bra_swapped_environment_in_heff(monkeypatch):
+    orig = od.apply_heff2
+    monkeypatch.setattr(od, "apply_heff2",
+                        lambda Lg, Rg, W1, W2, th: orig(Lg.transpose(2, 1, 0), Rg, W1, W2, th))
+    assert _caught(dmrg_pins.test_apply_heff2_matches_reference_einsum, False)
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_dropped_conjugate_in_environment_update(monkeypatch, side):
+    name = "update_%s_env" % side
+    orig = getattr(od, name)
+    monkeypatch.setattr(od, name, lambda E, A, W: orig(E, np.asarray(A).view(_NoConj), W))
+    assert _caught(dmrg_pins.test_update_envs_match_reference_einsums, True)
+    # control: on real tensors conj() is the identity, and the pin passes
+    assert not _caught(dmrg_pins.test_update_envs_match_reference_einsums, False)
+
+
+def test_swapped_mpo_bond_indices_in_environment_update(monkeypatch):
+    orig = od.update_right_env
+    monkeypatch.setattr(od, "update_right_env",
+                        lambda R, A, W: orig(R, A, W.transpose(3, 1, 2, 0)))
+    assert _caught(dmrg_pins.test_update_envs_match_reference_einsums, False)
+
+
+def _off_diagonal_scaled(Ws, f, every=1):
+    """Scale the entries with s != sp (the S+ and S- factors of the hopping
+    terms) by f on every `every`-th site."""
+    out = []
+    for i, W in enumerate(Ws):
+        W = W.copy()
+        if i % every:
+            out.append(W)
+            continue
+        d = W.shape[1]
+        for s in range(d):
+            for sp in range(d):
+                if s != sp:
+                    W[:, s, sp, :] *= f
+        out.append(W)
+    return out
+
+
+def test_sign_flipped_hopping_in_mpo(monkeypatch):
+    """A flipped hopping sign (one factor of each term negated: every other
+    site) is unitarily equivalent on a chain, by a rotation of every other
+    site, so energy checks alone pass it; the element-wise MPO = Kronecker
+    pin does not."""
+    orig = models.heisenberg_mpo
+    monkeypatch.setattr(models, "heisenberg_mpo",
+                        lambda L, J=1.0: _off_diagonal_scaled(orig(L, J), -1.0, every=2))
+    E_mut = models.ground_energy(models.mpo_to_dense(models.heisenberg_mpo(4)))
+    assert abs(E_mut - (-1.616025403784)) < 1e-10   # the spectrum does not see it
+    assert _caught(model_pins.test_heisenberg_mpo_equals_kronecker, 4)
+
+
+def test_dropped_hopping_in_mpo(monkeypatch):
+    orig = models.heisenberg_mpo
+    monkeypatch.setattr(models, "heisenberg_mpo", lambda L, J=1.0: _off_diagonal_scaled(orig(L, J), 0.0))
+    assert _caught(model_pins.test_heisenberg_mpo_equals_kronecker, 4)
+    assert _caught(dmrg_pins.test_dmrg2_known_answers, 4, -1.616025403784)
+
+
+def test_relative_instead_of_absolute_cutoff(monkeypatch):
+    orig = od.truncation_rank
+    monkeypatch.setattr(od, "truncation_rank",
+                        lambda S, k, cutoff=od.SVD_ABS_CUTOFF, rel_cutoff=0.0:
+                        orig(S, k, 0.0, max(cutoff, rel_cutoff)))
+    assert _caught(dmrg_pins.test_svd_split2_truncation_rule)
+
+
+def test_off_by_one_random_stream(monkeypatch):
+    orig = prng.splitmix64
+    monkeypatch.setattr(prng, "splitmix64", lambda seed, count, offset=0: orig(seed, count, offset + 1))
+    assert _caught(model_pins.test_splitmix64_published_vector)
