@@ -4,7 +4,7 @@ oracle.
 Each test plants one mistake in the oracle on the CPU (a transposed operand, a
 swapped index pair, a dropped conjugate, a dropped or sign-flipped term, a
 relative cutoff in place of the absolute one, an off-by-one in the random
-stream) by monkeypatching, then runs a pin from test_oracle_dmrg.py or
+stream, a conjugated flux phase) by monkeypatching, then runs a pin from test_oracle_dmrg.py or
 test_oracle_models.py unchanged and requires it to fail.  The unmutated pins
 pass in those modules.
 """
@@ -114,3 +114,20 @@ def test_off_by_one_random_stream(monkeypatch):
     orig = prng.splitmix64
     monkeypatch.setattr(prng, "splitmix64", lambda seed, count, offset=0: orig(seed, count, offset + 1))
     assert _caught(model_pins.test_splitmix64_published_vector)
+
+
+def test_conjugated_flux_phase_in_josephson_mpo(monkeypatch):
+    """The wrong sign of the flux phase gives conj(H): the same spectrum, so
+    the known answers pass it; the element-wise MPO = Kronecker pin does not."""
+    orig = models.josephson_mpo
+
+    def mutant(L, n_max=2, E_J=2.0, E_C=1.0, phi_ext=np.pi / 3, n_g=0.0):
+        return orig(L, n_max=n_max, E_J=E_J, E_C=E_C, phi_ext=-phi_ext, n_g=n_g)
+
+    monkeypatch.setattr(models, "josephson_mpo", mutant)
+    kw = dict(n_max=1, E_J=1.0, E_C=0.5, phi_ext=np.pi / 4)
+    E_mut = models.ground_energy(models.mpo_to_dense(models.josephson_mpo(4, **kw)))
+    assert abs(E_mut - (-1.053346829927396)) < 1e-10   # the spectrum does not see it
+    assert _caught(model_pins.test_josephson_mpo_equals_kronecker, 3)
+    # and the reference's own Josephson MPO fixes which sign is right
+    assert _caught(model_pins.test_fixture_josephson_mpo_matches_builder)
