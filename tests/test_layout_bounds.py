@@ -51,3 +51,17 @@ def test_views_are_bijections(kind, a, b, d, D):
     for arr in (ax, ay):
         assert arr.min() == 0 and arr.max() == size - 1
         assert np.unique(arr).size == size
+
+
+@pytest.mark.parametrize("cR", [1, 3, 16, 33])
+@pytest.mark.parametrize("d", [2, 3, 5])
+def test_warm_basis_right_view_is_a_bijection(cR, d):
+    """warm.hip warm_basis_right: the stored N x N eigenbasis X (N = d*cR) is
+    read and written as a (d*N) x cR operand, row i = s + d*j -> s + N*j,
+    column k -> d*k (the same view for the output).  Every address of the
+    N x N buffer is reached exactly once."""
+    N = d * cR
+    v = ((d, 1, N), (1 << 30, d, 0))
+    a = all_addrs(v, d * N, cR).ravel()
+    assert a.min() == 0 and a.max() == N * N - 1
+    assert np.unique(a).size == N * N == a.size
