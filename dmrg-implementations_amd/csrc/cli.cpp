@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/dmrg_amd.h"
@@ -274,6 +275,27 @@ static int check(int rc, const char* what) {
   return rc;
 }
 
+// --dump-mpo FILE: the MPO exactly as handed to dmrg_set_mpo, written before
+// any device call.  Layout: int64 L, D, d, complex flag; then per site the
+// D*d*d*D scalars of W[w + s*D + sp*D*d + wp*D*d*d] (float64 or complex128).
+template <class V>
+static int dump_mpo(const char* path, const std::vector<std::vector<V>>& W, int D, int d) {
+  FILE* f = fopen(path, "wb");
+  if (!f) {
+    fprintf(stderr, "ERROR: cannot write %s\n", path);
+    return 1;
+  }
+  const long long hdr[4] = {(long long)W.size(), D, d, std::is_same<V, cplx>::value ? 1 : 0};
+  bool ok = fwrite(hdr, 8, 4, f) == 4;
+  for (const auto& w : W) ok = ok && fwrite(w.data(), sizeof(V), w.size(), f) == w.size();
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) {
+    fprintf(stderr, "ERROR: writing %s failed\n", path);
+    return 1;
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   int L = 8, chi = 32, sweeps = 30, n_max = 1, krylov = 100, single = 0, fp32 = 0, init = -1;
   bool jj = false, tf = false, quiet = false, j12 = false, j123 = false, lad = false;
@@ -282,6 +304,7 @@ int main(int argc, char** argv) {
   unsigned long long seed = 1;
   const char* mpo_file = nullptr;
   const char* mps_file = nullptr;
+  const char* dump_file = nullptr;
   int pos = 0;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -314,6 +337,7 @@ int main(int argc, char** argv) {
     else if (a == "--seed") seed = strtoull(next(), nullptr, 10);
     else if (a == "--mpo") mpo_file = next();
     else if (a == "--mps") mps_file = next();
+    else if (a == "--dump-mpo") dump_file = next();
     else if (a.size() && a[0] == '-') { fprintf(stderr, "Unknown flag: %s\n", a.c_str()); return 1; }
     else if (pos == 0) { L = atoi(a.c_str()); ++pos; }
     else if (pos == 1) { chi = atoi(a.c_str()); ++pos; }
@@ -336,6 +360,7 @@ int main(int argc, char** argv) {
     }
     int D = 0, d = 0;
     std::vector<std::vector<cplx>> W = mpo_from_bin(tw, D, d);
+    if (dump_file) return dump_mpo(dump_file, W, D, d);
     const int Lf = (int)W.size();
     if (pos > 0 && L != Lf) { fprintf(stderr, "ERROR: L=%d but %s has %d sites\n", L, mpo_file, Lf); return 1; }
     if (mps_file && (int)ta.size() != Lf) { fprintf(stderr, "ERROR: MPS and MPO site counts differ\n"); return 1; }
@@ -375,6 +400,7 @@ int main(int argc, char** argv) {
   if (jj) {
     int d = 2 * n_max + 1;
     auto W = josephson(L, n_max, EJ, EC, phi);
+    if (dump_file) return dump_mpo(dump_file, W, 4, d);
     check(dmrg_create(L, d, chi, 4, 1e-10, DMRG_DTYPE_C128, variant, &o, &h), "create");
     std::vector<const void*> p(L);
     for (int i = 0; i < L; ++i) p[i] = W[i].data();
@@ -425,6 +451,7 @@ int main(int argc, char** argv) {
     ed = {{4, -1.616025403784}, {8, -3.374932598688}};
     pass_tol = 1e-10;
   }
+  if (dump_file) return dump_mpo(dump_file, W, D, d);
   check(dmrg_create(L, d, chi, D, 1e-10, DMRG_DTYPE_F64, variant, &o, &h), "create");
   std::vector<const void*> p(L);
   for (int i = 0; i < L; ++i) p[i] = W[i].data();
