@@ -173,3 +173,28 @@ def test_ladder_decoupled_rungs():
         assert abs(E - (-0.75 * 1.7 * L)) < 1e-12
         Em = models.ground_energy(models.mpo_to_dense(models.ladder_mpo(L, 0.0, 1.7)))
         assert abs(Em - E) < 1e-12
+
+
+def _tfim_free_fermion_energy(L, J, h):
+    """Open transverse-field Ising chain H = -J sum sz_i sz_{i+1} - h sum sx_i
+    (Pauli): by the Jordan-Wigner mapping to free fermions (Lieb, Schultz and
+    Mattis 1961; Pfeuty 1970) the ground energy is minus the sum of the
+    singular values of the L x L bidiagonal matrix with h on the diagonal and
+    J above it."""
+    M = np.diag(np.full(L, float(h))) + np.diag(np.full(L - 1, float(J)), 1)
+    return -np.linalg.svd(M, compute_uv=False).sum()
+
+
+@pytest.mark.parametrize("L", [2, 3, 5, 8, 10])
+@pytest.mark.parametrize("J,h", [(1.0, 1.0), (1.0, 0.3), (0.7, 1.9)])
+def test_tfim_ed_free_fermion_closed_form(L, J, h):
+    """Pins dense_tfim (Pauli operators, both signs, open ends): spin-1/2
+    operators in place of Pauli ones, or a missing bond, change E0."""
+    E = models.ground_energy(models.dense_tfim(L, J=J, hx=h))
+    assert abs(E - _tfim_free_fermion_energy(L, J, h)) < 1e-12 * L
+
+
+@pytest.mark.parametrize("L", [2, 3, 5])
+def test_tfim_mpo_equals_kronecker(L):
+    H = models.mpo_to_dense(models.tfim_mpo(L, J=0.8, hx=0.6))
+    assert np.abs(H - models.dense_tfim(L, J=0.8, hx=0.6).toarray()).max() < 1e-15

@@ -4,7 +4,8 @@ oracle.
 Each test plants one mistake in the oracle on the CPU (a transposed operand, a
 swapped index pair, a dropped conjugate, a dropped or sign-flipped term, a
 relative cutoff in place of the absolute one, an off-by-one in the random
-stream, a conjugated flux phase) by monkeypatching, then runs a pin from test_oracle_dmrg.py or
+stream, a conjugated flux phase, spin-1/2 operators in place of Pauli ones) by
+monkeypatching, then runs a pin from test_oracle_dmrg.py or
 test_oracle_models.py unchanged and requires it to fail.  The unmutated pins
 pass in those modules.
 """
@@ -131,3 +132,22 @@ def test_conjugated_flux_phase_in_josephson_mpo(monkeypatch):
     assert _caught(model_pins.test_josephson_mpo_equals_kronecker, 3)
     # and the reference's own Josephson MPO fixes which sign is right
     assert _caught(model_pins.test_fixture_josephson_mpo_matches_builder)
+
+
+def test_spin_half_operators_in_tfim(monkeypatch):
+    """S = sigma/2 in place of the Pauli matrices scales the bond term by 1/4
+    and the field by 1/2: the free-fermion closed form rejects it."""
+    orig = models.dense_tfim
+    monkeypatch.setattr(models, "dense_tfim", lambda L, J=1.0, hx=1.0: orig(L, J / 4, hx / 2))
+    assert _caught(model_pins.test_tfim_ed_free_fermion_closed_form, 5, 1.0, 0.3)
+
+
+def test_flipped_field_sign_in_tfim_mpo(monkeypatch):
+    """-hx for hx is unitarily equivalent (rotate every site about z), so only
+    the element-wise MPO = Kronecker pin sees it."""
+    orig = models.tfim_mpo
+    monkeypatch.setattr(models, "tfim_mpo", lambda L, J=1.0, hx=1.0: orig(L, J, -hx))
+    E_mut = models.ground_energy(models.mpo_to_dense(models.tfim_mpo(5, J=0.8, hx=0.6)))
+    E_ok = models.ground_energy(models.dense_tfim(5, J=0.8, hx=0.6))
+    assert abs(E_mut - E_ok) < 1e-12          # the spectrum does not see it
+    assert _caught(model_pins.test_tfim_mpo_equals_kronecker, 5)
