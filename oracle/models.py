@@ -21,8 +21,10 @@ Pinning
   and the ground energies compared with the reference's exact-diagonalisation
   known answers (test_dmrg2_gpu.cpp:208-211, :292-297).
 * ``dense_*``: pinned by those same known answers (Heisenberg L=4/8,
-  Josephson n_max=1 L=4/6) and by closed forms (two-site singlet -3/4,
-  ferromagnet (L-1)/4, TFIM h=0 / J=0 limits).
+  Josephson n_max=1 L=4/6) and by closed forms (two-site singlet -3/4, the
+  all-up eigenstate (L-1)/4; the transverse-field Ising chain by its
+  free-fermion ground energy, minus the singular-value sum of the bidiagonal
+  matrix with h on the diagonal and J above it).
 * ``spin_chain_mpo`` / ``ladder_mpo`` and ``dense_spin_chain`` /
   ``dense_ladder``: MPO against Kronecker form element-wise; ground energies
   against the reference's known answers (J1-J2 at the Majumdar-Ghosh point,

@@ -198,3 +198,16 @@ def test_tfim_ed_free_fermion_closed_form(L, J, h):
 def test_tfim_mpo_equals_kronecker(L):
     H = models.mpo_to_dense(models.tfim_mpo(L, J=0.8, hx=0.6))
     assert np.abs(H - models.dense_tfim(L, J=0.8, hx=0.6).toarray()).max() < 1e-15
+
+
+@pytest.mark.parametrize("L", [2, 3, 6])
+def test_dense_heisenberg_closed_forms(L):
+    """The all-up state is an exact eigenstate with E = (L-1)/4 (the top of the
+    spectrum); two sites have the singlet ground energy -3/4."""
+    H = models.dense_heisenberg(L).toarray()
+    up = np.zeros(2 ** L)
+    up[0] = 1.0
+    assert np.abs(H @ up - (L - 1) / 4 * up).max() == 0.0
+    assert np.linalg.eigvalsh(H)[-1] == pytest.approx((L - 1) / 4, abs=1e-13)
+    if L == 2:
+        assert models.ground_energy(models.dense_heisenberg(2)) == pytest.approx(-0.75, abs=1e-14)
