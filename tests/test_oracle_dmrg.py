@@ -212,3 +212,61 @@ def test_pdmrg_serial_matches_oracle(L):
     E, _ = ps.serial_dmrg([np.array(A) for A in mps], [ps.to_pdmrg_mpo(W) for W in Ws], 32, 6)
     E_ed = models.ground_energy(models.dense_heisenberg(L))
     assert abs(E - E_ed) < 1e-9
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_apply_heff1_is_heff2_with_an_identity_site(cplx):
+    """Single-site H_eff equals the (einsum-pinned) two-site one whose second
+    site is a d = 1 site carrying the MPO bond through unchanged."""
+    rng = np.random.default_rng(8)
+    cL, cR, d, D = 3, 4, 3, 4
+    Lg, Rg = _r(rng, (cL, D, cL), cplx), _r(rng, (cR, D, cR), cplx)
+    W = _r(rng, (D, d, d, D), cplx)
+    th = _r(rng, (cL, d, cR), cplx)
+    W_id = np.eye(D).reshape(D, 1, 1, D)
+    ref = od.apply_heff2(Lg, Rg, W, W_id, th[:, :, None, :])[:, :, 0, :]
+    got = od.apply_heff1(Lg, Rg, W, th)
+    assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("two_site", [False, True])
+def test_heff_is_hermitian_for_a_hermitian_mpo(two_site):
+    """Environments of a Hermitian MPO (Josephson, complex) on a random complex
+    MPS make H_eff Hermitian: <x, H y> = <H x, y>."""
+    L = 6
+    Ws = models.josephson_mpo(L, n_max=1, E_J=1.3, E_C=0.7, phi_ext=0.4)
+    D, d = Ws[0].shape[0], Ws[0].shape[1]
+    mps, b = od.random_mps(L, d, 8, seed=12, complex_=True)
+    Lg = od.left_boundary(D, complex)
+    for i in range(2):
+        Lg = od.update_left_env(Lg, mps[i], Ws[i])
+    last = 3 if two_site else 2
+    Rg = od.right_boundary(D, complex)
+    for i in range(L - 1, last, -1):
+        Rg = od.update_right_env(Rg, mps[i], Ws[i])
+    rng = np.random.default_rng(2)
+    shape = (b[2], d, d, b[4]) if two_site else (b[2], d, b[3])
+    x, y = _r(rng, shape, True), _r(rng, shape, True)
+    H = (lambda v: od.apply_heff2(Lg, Rg, Ws[2], Ws[3], v)) if two_site else \
+        (lambda v: od.apply_heff1(Lg, Rg, Ws[2], v))
+    lhs, rhs = np.vdot(x, H(y)), np.vdot(H(x), y)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("direction", ["R", "L"])
+def test_svd_split2_factors(direction):
+    """The orthonormal factor is an isometry, the kept singular values are
+    numpy's, and the product is the best rank-k approximation."""
+    rng = np.random.default_rng(4)
+    cL, d, cR, k = 5, 2, 6, 4
+    th = _r(rng, (cL, d, d, cR), True)
+    M = th.reshape(cL * d, d * cR, order="F")
+    s = np.linalg.svd(M, compute_uv=False)
+    Al, Ar, S, disc = od.svd_split2(th, k, direction)
+    assert np.abs(S - s[:k]).max() < 1e-13
+    A = Al.reshape(cL * d, k, order="F")
+    B = Ar.reshape(k, d * cR, order="F")
+    Q = A if direction == "R" else B.conj().T
+    assert np.abs(Q.conj().T @ Q - np.eye(k)).max() < 1e-13
+    assert abs(np.linalg.norm(M - A @ B) ** 2 - np.sum(s[k:] ** 2)) < 1e-12
+    assert abs(disc - np.sum(s[k:] ** 2)) < 1e-12

@@ -151,3 +151,14 @@ def test_flipped_field_sign_in_tfim_mpo(monkeypatch):
     E_ok = models.ground_energy(models.dense_tfim(5, J=0.8, hx=0.6))
     assert abs(E_mut - E_ok) < 1e-12          # the spectrum does not see it
     assert _caught(model_pins.test_tfim_mpo_equals_kronecker, 5)
+
+
+def test_transposed_mpo_operand_in_single_site_heff(monkeypatch):
+    """For a Hermitian MPO a transposed operand gives conj(H_eff), still
+    Hermitian; the identity-site relation to the einsum-pinned two-site H_eff
+    (random, non-Hermitian tensors) rejects it."""
+    orig = od.apply_heff1
+    monkeypatch.setattr(od, "apply_heff1",
+                        lambda Lg, Rg, W, th: orig(Lg, Rg, W.transpose(0, 2, 1, 3), th))
+    assert not _caught(dmrg_pins.test_heff_is_hermitian_for_a_hermitian_mpo, False)
+    assert _caught(dmrg_pins.test_apply_heff1_is_heff2_with_an_identity_site, True)
