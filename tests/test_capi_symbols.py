@@ -62,3 +62,63 @@ def test_batch_refuses_an_engine_listed_twice(dmrg_amd):
     rc = lib.dmrg_run_batch(hs, 3, 1, 2, C.cast(E, C.c_void_p), None)
     assert rc == dmrg_amd.ERR_BAD_ARG
     assert b"twice" in lib.dmrg_last_error()
+
+
+def _header_prototypes():
+    """name -> (return type, [parameter types]) from include/dmrg_amd.h."""
+    hdr = open(os.path.join(ROOT, "include", "dmrg_amd.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", " ", hdr, flags=re.S)
+    protos = {}
+    for m in re.finditer(r"(const char\s*\*|int|void)\s+(dmrg_\w+)\s*\(([^)]*)\)\s*;", hdr):
+        params = [p.strip() for p in m.group(3).split(",") if p.strip() and p.strip() != "void"]
+        types = [re.sub(r"\s*\b\w+$", "", p) if not p.endswith("*") else p for p in params]
+        protos[m.group(2)] = (m.group(1).replace(" ", ""), types)
+    return protos
+
+
+def _kind_c(t):
+    t = t.replace("const ", "").strip()
+    if "*" in t or t in ("dmrg_handle", "dmrg_ctx_handle"):
+        return "ptr"
+    return {"int": "int", "double": "double", "uint64_t": "u64", "size_t": "u64"}[t]
+
+
+def _kind_ctypes(t):
+    import ctypes as C
+    if t is None:
+        return "void"
+    if t in (C.c_void_p, C.c_char_p) or (isinstance(t, type) and issubclass(t, C._Pointer)):
+        return "ptr"
+    if t is C.c_int:
+        return "int"
+    if t is C.c_double:
+        return "double"
+    if t in (C.c_uint64, C.c_size_t):
+        return "u64"
+    raise AssertionError("unexpected ctypes type %r" % (t,))
+
+
+def test_ctypes_signatures_match_the_header(dmrg_amd):
+    """Every prototype of the header against the binding's ctypes table:
+    argument count and kind (pointer, int, double, 64-bit unsigned), and the
+    return kind, so a drifted signature cannot pass arguments silently wrong."""
+    lib = dmrg_amd.lib()
+    protos = _header_prototypes()
+    assert set(dmrg_amd.EXPORTED) <= set(protos)
+    for name, (ret, params) in protos.items():
+        f = getattr(lib, name)
+        assert f.argtypes is not None, name
+        got = [_kind_ctypes(t) for t in f.argtypes]
+        assert got == [_kind_c(p) for p in params], (name, got, params)
+        want_ret = {"int": "int", "void": "void", "constchar*": "ptr"}[ret]
+        assert _kind_ctypes(f.restype) == want_ret, name
+
+
+def test_missing_library_raises(dmrg_amd, monkeypatch, tmp_path):
+    """No fallback: without the built library the binding raises."""
+    import pytest
+
+    monkeypatch.setattr(dmrg_amd, "_lib", None)
+    monkeypatch.setattr(dmrg_amd, "LIB_PATH", str(tmp_path / "libdmrg_amd.so"))
+    with pytest.raises(ImportError):
+        dmrg_amd.lib()
