@@ -104,7 +104,9 @@ typedef struct dmrg_stats {
 
 typedef struct dmrg_engine* dmrg_handle;
 
-/* Fill `opt` with the reference's hard-coded defaults. */
+/* Fill `opt` with the reference's hard-coded defaults: Krylov cap 100
+ * (gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h:138), Lanczos tolerances 1e-12
+ * (:975-976), SVD cutoff 1e-14 (:1276); the other fields are this engine's. */
 void dmrg_default_options(dmrg_options* opt);
 
 /* Replaces DMRG2GPU(int L, int d, int chi_max, int D_mpo, double tol)
@@ -158,7 +160,8 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
 /* Replaces get_energy() (dmrg2_gpu.h:40). */
 int dmrg_get_energy(dmrg_handle h, double* energy);
 /* Energies at the end of each completed sweep: the count in *n, the first
- * min(cap, *n) of them in out (which may be NULL when cap <= 0). */
+ * min(cap, *n) of them in out (which may be NULL when cap <= 0).  No
+ * reference counterpart: run keeps only the last one (dmrg2_gpu_impl.h:1440-1446). */
 int dmrg_get_sweep_energies(dmrg_handle h, double* out, int cap, int* n);
 /* Replaces chi_L / chi_R (dmrg2_gpu.h:42-43): dims[0..L] of all bonds. */
 int dmrg_get_bond_dims(dmrg_handle h, int* dims);
@@ -176,9 +179,12 @@ int dmrg_expectation(dmrg_handle h, double* energy, double* norm2);
 /* Per-phase timings and counters; no reference counterpart beyond the
  * DMRG_GPU_PROFILE phase timers (gpu-rocm/common/gpu_opts.h:57-93). */
 int dmrg_get_stats(dmrg_handle h, dmrg_stats* st);
+/* Zero those counters (no reference counterpart). */
 int dmrg_reset_stats(dmrg_handle h);
 
-/* Message of the last failure on this thread ("" if none). */
+/* Message of the last failure on this thread ("" if none).  No reference
+ * counterpart: the reference prints the message and throws
+ * (gpu-rocm/common/hip_check.h:13-21). */
 const char* dmrg_last_error(void);
 
 /* ------------------------------------------------------------------------
@@ -188,6 +194,9 @@ const char* dmrg_last_error(void);
  * and names the device, which each call makes current on its thread.
  * ---------------------------------------------------------------------- */
 typedef struct dmrg_ctx* dmrg_ctx_handle;
+/* Create / destroy an op-level context.  No reference counterpart: the
+ * reference keeps its workspaces inside the engine object
+ * (gpu-rocm/dmrg2-gpu/src/dmrg2_gpu.h:95-140). */
 int dmrg_ctx_create(int device, dmrg_ctx_handle* out);
 void dmrg_ctx_destroy(dmrg_ctx_handle c);
 
@@ -247,14 +256,17 @@ int dmrg_op_gemm(dmrg_ctx_handle c, void* stream, int dtype, int opA, int opB, i
 int dmrg_op_gemm_f32(dmrg_ctx_handle c, void* stream, int prec, int opA, int opB, int M, int N,
                      int K, const float* A, int lda, const float* B, int ldb, float* Cm, int ldc);
 
-/* Device-memory helpers so callers need no other HIP binding. */
+/* Device-memory helpers so callers need no other HIP binding (no reference
+ * counterpart: the reference's callers never hold device memory). */
 int dmrg_dev_malloc(void** p, size_t bytes);
 int dmrg_dev_free(void* p);
 int dmrg_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int dmrg_memcpy_d2h(void* dst, const void* src, size_t bytes);
 /* Device-to-device copy (a copy kernel of known size; the traffic
- * calibration of the profiling runs uses it). */
+ * calibration of the profiling runs uses it).  No reference counterpart. */
 int dmrg_memcpy_d2d(void* dst, const void* src, size_t bytes);
+/* Wait for all device work; visible HIP devices (0 without a GPU).  No
+ * reference counterpart. */
 int dmrg_dev_synchronize(void);
 int dmrg_device_count(int* n);
 

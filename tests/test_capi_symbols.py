@@ -122,3 +122,19 @@ def test_missing_library_raises(dmrg_amd, monkeypatch, tmp_path):
     monkeypatch.setattr(dmrg_amd, "LIB_PATH", str(tmp_path / "libdmrg_amd.so"))
     with pytest.raises(ImportError):
         dmrg_amd.lib()
+
+
+def test_every_entry_point_cites_the_reference_or_says_it_has_none():
+    """Each prototype's comment cites the reference interface it replaces
+    (file:line) or states that it has no counterpart.  A prototype that
+    directly follows another, with no comment of its own, shares that one's."""
+    hdr = open(os.path.join(ROOT, "include", "dmrg_amd.h")).read()
+    prev, last, missing, n = 0, "", [], 0
+    for m in re.finditer(r"(const char\s*\*|int|void)\s+(dmrg_\w+)\s*\(([^)]*)\)\s*;", hdr):
+        comments = re.findall(r"/\*.*?\*/", hdr[prev:m.start()], flags=re.S)
+        c = comments[-1] if comments else last
+        if not (re.search(r"\.(h|hpp|cpp|py):\d+", c) or "counterpart" in c.lower()):
+            missing.append(m.group(2))
+        last, prev, n = c, m.end(), n + 1
+    assert n == len(_header_prototypes())
+    assert not missing, missing
