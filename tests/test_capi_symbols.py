@@ -138,3 +138,9 @@ def test_every_entry_point_cites_the_reference_or_says_it_has_none():
         last, prev, n = c, m.end(), n + 1
     assert n == len(_header_prototypes())
     assert not missing, missing
+
+
+def test_integration_guide_names_every_entry_point():
+    guide = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    missing = [f for f in sorted(_header_prototypes()) if f not in guide]
+    assert not missing, missing
