@@ -64,8 +64,8 @@ def test_c_program_links_and_calls(tmp_path):
 
 
 def test_integration_cxx_example_compiles(tmp_path):
-    """The C++ caller shown in INTEGRATION.md section 2 compiles against the
-    header as written (wrapped in a function: it is a fragment)."""
+    """The C++ caller shown in the integration guide (its section 2) compiles
+    against the header as written (wrapped in a function: it is a fragment)."""
     _need("g++")
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     sec = text[text.index("## 2. C++ caller"):]
@@ -85,10 +85,11 @@ def test_integration_cxx_example_compiles(tmp_path):
 
 
 def test_integration_ctypes_stub_runs(monkeypatch):
-    """The ctypes stub shown in INTEGRATION.md section 3 runs as written.  With
-    L = 1 dmrg_create refuses the chain before any device call, so every
-    later call sees a NULL handle and returns BAD_ARG; what this checks is
-    that the stub's argument types and the options buffer fit the ABI."""
+    """The ctypes stub shown in the integration guide (its section 3) runs as
+    written.  With L = 1 dmrg_create refuses the chain before any device
+    call, so every later call sees a NULL handle and returns BAD_ARG; what
+    this checks is that the stub's argument types and the options buffer fit
+    the ABI."""
     import numpy as np
 
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
