@@ -60,3 +60,29 @@ def test_batch_chains_are_seeded_and_uneven():
         assert L in (16, 24, 32, 48, 64) and chi in (32, 64, 128, 256)
         assert h.shape == (L,) and np.all(np.abs(h) <= W)
     assert len({(L, chi) for L, chi, _, _ in chains}) > 8   # uneven sizes
+
+
+def test_traffic_figure_comes_from_the_committed_profile():
+    import json
+    import os
+
+    path = os.path.join(bench.ROOT, "profiles", "c4_heff_traffic.json")
+    want = json.load(open(path))["bytes_per_matvec"]
+    assert bench.heff_traffic("c4", "fp32") == want
+    assert 1e8 < want < 1e9                       # the C4 centre-bond matvec: ~0.19 GB
+    assert bench.heff_traffic("c4", "fp64") is None
+    assert bench.heff_traffic("c2", "fp32") is None
+
+
+def test_profile_summaries_name_their_commands():
+    """Each rocprofv3 summary gives the command that made it, run on committed
+    files (bench.py), never a scratch script."""
+    import glob
+    import os
+
+    files = glob.glob(os.path.join(bench.ROOT, "profiles", "*.md"))
+    assert files
+    for f in files:
+        text = open(f).read()
+        assert "rocprofv3" in text and "bench.py" in text, f
+        assert "_work" not in text, f
