@@ -322,6 +322,7 @@ struct CtxT {
     if (sc.X) DMRG_HIP(hipFree(sc.X));
     if (sc.Y) DMRG_HIP(hipFree(sc.Y));
     sc.X = sc.Y = nullptr;
+    sc.cap = 0;  // a failed allocation below leaves no stale capacity
     DMRG_HIP(hipMalloc(&sc.X, n * sizeof(T)));
     DMRG_HIP(hipMalloc(&sc.Y, n * sizeof(T)));
     sc.cap = n;
@@ -330,6 +331,7 @@ struct CtxT {
     if (n > buf_cap) {
       if (buf) DMRG_HIP(hipFree(buf));
       buf = nullptr;
+      buf_cap = 0;
       DMRG_HIP(hipMalloc(&buf, n * sizeof(T)));
       buf_cap = n;
     }
@@ -343,12 +345,44 @@ struct CtxT {
     if (lz.kmax < kmax || lz.part_bytes < pb) lz.alloc(std::max(kmax, 128), std::max(pb, lz.part_bytes), sizeof(T));
   }
 };
+// Scratch of the FP32 ops: contraction intermediates and the fused MPO.
+struct CtxF32 {
+  ContractScratch<float> sc;
+  float* buf = nullptr;
+  size_t buf_cap = 0;
+  ~CtxF32() {
+    if (sc.X) (void)hipFree(sc.X);
+    if (sc.Y) (void)hipFree(sc.Y);
+    if (buf) (void)hipFree(buf);
+  }
+  void need_sc(size_t n) {
+    if (n <= sc.cap) return;
+    if (sc.X) DMRG_HIP(hipFree(sc.X));
+    if (sc.Y) DMRG_HIP(hipFree(sc.Y));
+    sc.X = sc.Y = nullptr;
+    sc.cap = 0;
+    DMRG_HIP(hipMalloc(&sc.X, n * sizeof(float)));
+    DMRG_HIP(hipMalloc(&sc.Y, n * sizeof(float)));
+    sc.cap = n;
+  }
+  float* need_buf(size_t n) {
+    if (n > buf_cap) {
+      if (buf) DMRG_HIP(hipFree(buf));
+      buf = nullptr;
+      buf_cap = 0;
+      DMRG_HIP(hipMalloc(&buf, n * sizeof(float)));
+      buf_cap = n;
+    }
+    return buf;
+  }
+};
 }  // namespace
 
 struct dmrg_ctx {
   int device = 0;
   CtxT<double> r;
   CtxT<cdouble> c;
+  CtxF32 f;  // the FP32 matvec of the mixed modes
   int* h_status = nullptr;
   double* h_scal = nullptr;
   ~dmrg_ctx() {
@@ -411,6 +445,9 @@ int dmrg_op_heff2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, in
       using T = decltype(tag);
       auto& x = ctx_of<T>(c);
       hipStream_t st = (hipStream_t)stream;
+      // the caller's pending writes of the MPO and the previous call's reads
+      // of the context's MPO buffer are ordered on the stream: drain it first
+      DMRG_HIP(hipStreamSynchronize((hipStream_t)stream));
       const size_t wsz = (size_t)D * d * d * D, K = (size_t)D * d * d;
       auto h1 = d2h<T>(W1, wsz), h2 = d2h<T>(W2, wsz);
       std::vector<T> ww(K * K);
@@ -420,6 +457,33 @@ int dmrg_op_heff2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, in
       x.need_sc((size_t)D * d * d * cL * cR);
       heff2_apply<T>(st, (const T*)L, (const T*)R, dww, (const T*)theta, (T*)out, cL, cR, d, D, x.sc);
     });
+  });
+}
+
+int dmrg_op_heff2_f32(dmrg_ctx_handle c, void* stream, int prec, int cL, int cR, int d, int D,
+                      const float* L, const float* R, const double* W1, const double* W2,
+                      const float* theta, float* out) {
+  if (!c || !L || !R || !W1 || !W2 || !theta || !out) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
+  return guard([&] {
+    use(c);
+    DMRG_REQUIRE(cL >= 1 && cR >= 1 && d >= 1 && D >= 1, "bad shape");
+    DMRG_REQUIRE(prec == DMRG_PREC_FP32 || prec == DMRG_PREC_BF16X3, "prec must be FP32 or BF16X3");
+    hipStream_t st = (hipStream_t)stream;
+    // the caller's pending writes of the MPO and the previous call's reads
+    // of the context's MPO buffer are ordered on the stream: drain it first
+    DMRG_HIP(hipStreamSynchronize(st));
+    auto& x = c->f;
+    const size_t wsz = (size_t)D * d * d * D, K = (size_t)D * d * d;
+    auto h1 = d2h<double>(W1, wsz), h2 = d2h<double>(W2, wsz);
+    std::vector<double> ww(K * K);
+    build_ww2<double>(h1.data(), h2.data(), d, D, ww.data());
+    std::vector<float> wwf(K * K);
+    for (size_t i = 0; i < K * K; ++i) wwf[i] = (float)ww[i];
+    float* dww = x.need_buf(K * K);
+    DMRG_HIP(hipMemcpy(dww, wwf.data(), K * K * sizeof(float), hipMemcpyHostToDevice));
+    x.need_sc((size_t)D * d * d * cL * cR);
+    x.sc.split_bf16 = prec == DMRG_PREC_BF16X3;
+    heff2_apply<float>(st, L, R, dww, theta, out, cL, cR, d, D, x.sc);
   });
 }
 
@@ -463,6 +527,9 @@ int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co
     by_dtype(dtype, [&](auto tag) {
       using T = decltype(tag);
       auto& x = ctx_of<T>(c);
+      // the caller's pending writes of the MPO and the previous call's reads
+      // of the context's MPO buffer are ordered on the stream: drain it first
+      DMRG_HIP(hipStreamSynchronize((hipStream_t)stream));
       const size_t wsz = (size_t)D * d * d * D;
       auto hw = d2h<T>(W, wsz);
       std::vector<T> wr(wsz);
@@ -546,6 +613,9 @@ int dmrg_op_lanczos2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR,
       auto& x = ctx_of<T>(c);
       hipStream_t st = (hipStream_t)stream;
       const size_t n = (size_t)cL * d * d * cR;
+      // the caller's pending writes of the MPO and the previous call's reads
+      // of the context's MPO buffer are ordered on the stream: drain it first
+      DMRG_HIP(hipStreamSynchronize((hipStream_t)stream));
       const size_t wsz = (size_t)D * d * d * D, K = (size_t)D * d * d;
       auto h1 = d2h<T>(W1, wsz), h2 = d2h<T>(W2, wsz);
       std::vector<T> ww(K * K);
@@ -557,12 +627,14 @@ int dmrg_op_lanczos2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR,
       if ((size_t)kmax * n > x.V_cap) {
         if (x.V) DMRG_HIP(hipFree(x.V));
         x.V = nullptr;
+        x.V_cap = 0;
         DMRG_HIP(hipMalloc(&x.V, (size_t)kmax * n * sizeof(T)));
         x.V_cap = (size_t)kmax * n;
       }
       if (n > x.work_cap) {
         if (x.work) DMRG_HIP(hipFree(x.work));
         x.work = nullptr;
+        x.work_cap = 0;
         DMRG_HIP(hipMalloc(&x.work, n * sizeof(T)));
         x.work_cap = n;
       }

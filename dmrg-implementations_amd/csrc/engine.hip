@@ -190,6 +190,21 @@ void Engine<T>::alloc_all() {
   if (variant == 2 && opt.svd_warm) warm_alloc();
   DMRG_HIP(hipHostMalloc((void**)&h_status, 4 * sizeof(int), hipHostMallocDefault));
   DMRG_HIP(hipHostMalloc((void**)&h_scal, 8 * sizeof(double), hipHostMallocDefault));
+  DMRG_HIP(hipHostMalloc((void**)&h_aux, 8 * sizeof(T), hipHostMallocDefault));
+  // <psi|H|psi> contraction: two environment buffers and the identity MPO
+  dmalloc((void**)&ex_a, env_sz * sizeof(T));
+  dmalloc((void**)&ex_b, env_sz * sizeof(T));
+  dmalloc((void**)&ex_id, (size_t)d * d * sizeof(T));
+  {
+    std::vector<T> idm((size_t)d * d, T(0));
+    for (int s = 0; s < d; ++s) idm[s + (size_t)d * s] = T(1);
+    DMRG_HIP(hipMemcpy(ex_id, idm.data(), idm.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  if (opt.profile) {
+    // start/end pair per environment update of one sweep (2 (L-1) updates)
+    env_ev.resize(4 * (size_t)L);
+    for (auto& e : env_ev) DMRG_HIP(hipEventCreate(&e));
+  }
   runner.lz = &lz;
   runner.timed = opt.profile != 0;
   runner.V = V;
@@ -229,8 +244,14 @@ void Engine<T>::free_all() {
   runner.ev.clear();
   if (h_status) (void)hipHostFree(h_status);
   if (h_scal) (void)hipHostFree(h_scal);
+  if (h_aux) (void)hipHostFree(h_aux);
   h_status = nullptr;
   h_scal = nullptr;
+  h_aux = nullptr;
+  f(ex_a); f(ex_b); f(ex_id);
+  ex_a = ex_b = ex_id = nullptr;
+  for (auto e : env_ev) (void)hipEventDestroy(e);
+  env_ev.clear();
   if (ev_canon) (void)hipEventDestroy(ev_canon);
   if (ev_env) (void)hipEventDestroy(ev_env);
   if (s_main) (void)hipStreamDestroy(s_main);
@@ -352,16 +373,40 @@ void Engine<T>::get_mps(int site, T* out) const {
 // ---------------------------------------------------------------------------
 template <class T>
 void Engine<T>::env_left_at(int site) {
+  const bool timed = env_timing && env_ev_n + 2 <= env_ev.size();
+  if (timed) DMRG_HIP(hipEventRecord(env_ev[env_ev_n], s_env));
   env_left<T>(s_env, Lenv[site], mps[site], Wd[site], Lenv[site + 1], bond[site], bond[site + 1], d,
               D, sc_env);
+  if (timed) {
+    DMRG_HIP(hipEventRecord(env_ev[env_ev_n + 1], s_env));
+    env_ev_n += 2;
+  }
   stats.env_flop += flops_env(bond[site], bond[site + 1], d, D, is_cplx<T>::value);
 }
 
 template <class T>
 void Engine<T>::env_right_at(int site) {
+  const bool timed = env_timing && env_ev_n + 2 <= env_ev.size();
+  if (timed) DMRG_HIP(hipEventRecord(env_ev[env_ev_n], s_env));
   env_right<T>(s_env, Renv[site + 1], mps[site], Wrd[site], Renv[site], bond[site + 1], bond[site], d,
                D, sc_env);
+  if (timed) {
+    DMRG_HIP(hipEventRecord(env_ev[env_ev_n + 1], s_env));
+    env_ev_n += 2;
+  }
   stats.env_flop += flops_env(bond[site + 1], bond[site], d, D, is_cplx<T>::value);
+}
+
+// Adds the device time of the environment updates recorded since the last
+// call (option profile) to stats.env_ms; the streams must be synchronised.
+template <class T>
+void Engine<T>::flush_env_timing() {
+  for (size_t j = 0; j + 1 < env_ev_n; j += 2) {
+    float t = 0.f;
+    DMRG_HIP(hipEventElapsedTime(&t, env_ev[j], env_ev[j + 1]));
+    stats.env_ms += t;
+  }
+  env_ev_n = 0;
 }
 
 template <class T>
@@ -432,10 +477,10 @@ double Engine<T>::rayleigh2(int site) {
   matvec2(site, theta, work, true);
   lz_alpha<T>(s_main, lz, theta, work, n, 0);
   lz_alpha_value(s_main, lz, n, 0);
-  double a = 0.0;
-  DMRG_HIP(hipMemcpyAsync(&a, lz.alpha, sizeof(double), hipMemcpyDeviceToHost, s_main));
+  double* const ha = reinterpret_cast<double*>(h_aux);
+  DMRG_HIP(hipMemcpyAsync(ha, lz.alpha, sizeof(double), hipMemcpyDeviceToHost, s_main));
   DMRG_HIP(hipStreamSynchronize(s_main));
-  return a;
+  return ha[0];
 }
 
 template <class T>
@@ -487,6 +532,8 @@ void Engine<T>::warm_alloc() {
   for (T** p : {&wT[0], &wT[1], &wZ0, &wZ1, &wP, &wTmp}) dmalloc((void**)p, tsz * sizeof(T));
   dmalloc((void**)&wXw, capN * capN * sizeof(T));
   dmalloc((void**)&wDev, (size_t)kPolarMaxIter * kPolarParts * sizeof(double));
+  DMRG_HIP(hipHostMalloc((void**)&wHost, (size_t)kPolarMaxIter * kPolarParts * sizeof(double),
+                         hipHostMallocDefault));
   warm_on = true;
 }
 
@@ -505,6 +552,8 @@ void Engine<T>::warm_free() {
   }
   f(wDev);
   wDev = nullptr;
+  if (wHost) (void)hipHostFree(wHost);
+  wHost = nullptr;
   warm_on = false;
 }
 
@@ -512,9 +561,7 @@ void Engine<T>::warm_free() {
 template <class T>
 void Engine<T>::warm_begin() {
   if (!warm_on) return;
-  const T one = T(1);
-  DMRG_HIP(hipMemcpyAsync(wT[wCur], &one, sizeof(T), hipMemcpyHostToDevice, s_main));
-  DMRG_HIP(hipStreamSynchronize(s_main));
+  set_boundary_env<T>(s_main, wT[wCur], 1, 0);  // the 1 x 1 matrix [1], written on the device
   wTr = wTc = 1;
 }
 
@@ -528,7 +575,7 @@ const T* Engine<T>::warm_basis(int site, char dir) {
   // any T with singular values in (0, 1] has a polar factor; the basis it
   // gives is exact, and close to the new eigenbasis where the block changed
   // little, so accept every convergent start
-  if (!polar_unitary<T>(s_main, wT[wCur], cb, wZ0, wZ1, wP, wDev, 1.01, &Z)) return nullptr;
+  if (!polar_unitary<T>(s_main, wT[wCur], cb, wZ0, wZ1, wP, wDev, wHost, 1.01, &Z)) return nullptr;
   if (dir == 'R') warm_basis_left<T>(s_main, Z, cb, d, wX[wi][site], wXw);
   else warm_basis_right<T>(s_main, Z, cb, d, wX[wi][site], wXw);
   return wXw;
@@ -729,6 +776,12 @@ double Engine<T>::run(int n_sweeps, bool quiet) {
   stats.env_build_s = env_time;
   if (!quiet) printf("  Environment build: %.3f s\n", env_time);
   double e_prev = 0.0;
+  env_timing = !env_ev.empty();  // sweep updates only; the build above is env_build_s
+  env_ev_n = 0;
+  struct TimingOff {
+    Engine* e;
+    ~TimingOff() { e->env_timing = false; }
+  } timing_off{this};
   for (int sweep = 0; sweep < n_sweeps; ++sweep) {
     double ts = now_s();
     if (variant == 2) {
@@ -741,6 +794,7 @@ double Engine<T>::run(int n_sweeps, bool quiet) {
     DMRG_HIP(hipStreamSynchronize(s_main));
     DMRG_HIP(hipStreamSynchronize(s_env));
     stats.sweep_s_last = now_s() - ts;
+    flush_env_timing();
     stats.sweeps_s_total += stats.sweep_s_last;
     stats.sweeps_done += 1;
     sweep_E.push_back(energy);
@@ -766,25 +820,14 @@ template <class T>
 void Engine<T>::expectation(double* e, double* nrm2) {
   if (!mpo_set) throw Error(ST_STATE, "expectation: set_mpo has not been called");
   check_shapes("expectation");
-  DMRG_HIP(hipStreamSynchronize(s_main));
-  // Use Lenv[1..] as scratch is not allowed (they hold state); ping-pong in
-  // the spare MPS-sized buffers instead: env (chi, D, chi) fits in work/theta
-  // only when D <= d*d (two-site) -- use dedicated temporaries.
-  const size_t esz = (size_t)chi * D * chi;
-  T *ea = nullptr, *eb = nullptr, *wid = nullptr;
-  auto release = [&] {
-    if (ea) (void)hipFree(ea);
-    if (eb) (void)hipFree(eb);
-    if (wid) (void)hipFree(wid);
-  };
-  OnExit<decltype(release)> release_at_exit{release};
-  DMRG_HIP(hipMalloc(&ea, esz * sizeof(T)));
-  DMRG_HIP(hipMalloc(&eb, esz * sizeof(T)));
-  DMRG_HIP(hipMalloc(&wid, (size_t)d * d * sizeof(T)));
-  std::vector<T> idm((size_t)d * d, T(0));
-  for (int s = 0; s < d; ++s) idm[s + (size_t)d * s] = T(1);
-  DMRG_HIP(hipMemcpy(wid, idm.data(), idm.size() * sizeof(T), hipMemcpyHostToDevice));
-  T hv[2];
+  DMRG_HIP(hipSetDevice(opt.device));
+  DMRG_HIP(hipStreamSynchronize(s_env));
+  // the environments hold sweep state: contract in the two buffers set aside
+  // at creation (ex_a, ex_b) with the identity MPO ex_id for the norm
+  T* ea = ex_a;
+  T* eb = ex_b;
+  const T* wid = ex_id;
+  T* hv = h_aux;
   for (int pass = 0; pass < 2; ++pass) {
     const int Dp = pass == 0 ? D : 1;
     set_boundary_env<T>(s_main, ea, Dp, 0);
@@ -793,7 +836,7 @@ void Engine<T>::expectation(double* e, double* nrm2) {
                   sc_main);
       std::swap(ea, eb);
     }
-    DMRG_HIP(hipMemcpyAsync(&hv[pass], ea + (Dp - 1), sizeof(T), hipMemcpyDeviceToHost, s_main));
+    DMRG_HIP(hipMemcpyAsync(hv + pass, ea + (Dp - 1), sizeof(T), hipMemcpyDeviceToHost, s_main));
     DMRG_HIP(hipStreamSynchronize(s_main));
   }
   *nrm2 = real(hv[1]);

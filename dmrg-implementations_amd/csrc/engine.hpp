@@ -119,6 +119,10 @@ class Engine {
   void wait_env();
   void env_left_at(int site);
   void env_right_at(int site);  // builds R[site] from R[site+1] and A[site]
+  void flush_env_timing();
+  std::vector<hipEvent_t> env_ev;  // option profile: event pairs around env updates
+  size_t env_ev_n = 0;
+  bool env_timing = false;
   double optimize_bond(int site, char dir, bool rq);
   double optimize_site(int site, char dir, bool split);
   double sweep2_lr();
@@ -170,9 +174,12 @@ class Engine {
   T* wT[2] = {nullptr, nullptr};   // running block overlap (ping-pong)
   T *wZ0 = nullptr, *wZ1 = nullptr, *wP = nullptr, *wTmp = nullptr, *wXw = nullptr;
   double* wDev = nullptr;
+  double* wHost = nullptr;  // pinned read-back of the polar iteration's deviation
   int wTr = 0, wTc = 0, wCur = 0;
   int* h_status = nullptr;
   double* h_scal = nullptr;
+  T* h_aux = nullptr;  // pinned [8]: Rayleigh quotient and expectation read-backs
+  T *ex_a = nullptr, *ex_b = nullptr, *ex_id = nullptr;  // expectation(): envs, identity MPO
   size_t nmax = 0;
   int kmax = 0;
 };

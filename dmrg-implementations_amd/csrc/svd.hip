@@ -984,8 +984,8 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
     hipLaunchKernelGGL(k_norms_fin, dim3(1), dim3(256), 0, st, (const double*)w.red, nbr, w.scal);
   }
   DMRG_LAUNCH_CHECK();
-  double hs[3] = {0, 0, 0};
-  DMRG_HIP(hipMemcpyAsync(hs, w.scal, sizeof(hs), hipMemcpyDeviceToHost, st));
+  double* const hs = w.host;  // pinned: the copy engine writes it directly
+  DMRG_HIP(hipMemcpyAsync(hs, w.scal, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
   DMRG_HIP(hipStreamSynchronize(st));
   const double fro = hs[0];
   if (!(fro == fro) || std::isinf(fro)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
@@ -1003,7 +1003,6 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
   }
   double key_hi = 0.0, key_keep = 0.0;  // largest and keep-th largest diagonal entry
   double* const offpart = w.red + 2 * NRED;  // per-workgroup maxima of k_offmax
-  static thread_local std::vector<double> col_host;  // host copy of w.colpart
   auto offmax = [&](const T* M) {
     if (keep > 0)
       hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 64)), dim3(256), 0, st, M, N, Np, keep,
@@ -1011,8 +1010,9 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
     hipLaunchKernelGGL(k_offmax<T>, dim3(NRED), dim3(256), 0, st, M, N, Np,
                        keep > 0 ? (const int*)w.rank : nullptr, keep, offpart);
     DMRG_LAUNCH_CHECK();
-    double hp[NRED], kk[2] = {0.0, 0.0};
-    DMRG_HIP(hipMemcpyAsync(hp, offpart, sizeof(hp), hipMemcpyDeviceToHost, st));
+    double* const hp = w.host;         // NRED workgroup maxima
+    double* const kk = w.host + NRED;  // largest and keep-th largest diagonal entry
+    DMRG_HIP(hipMemcpyAsync(hp, offpart, NRED * sizeof(double), hipMemcpyDeviceToHost, st));
     if (keep > 0)
       DMRG_HIP(hipMemcpyAsync(kk, w.red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     DMRG_HIP(hipStreamSynchronize(st));
@@ -1021,7 +1021,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
       key_keep = kk[1];
     }
     double o = 0.0;
-    for (double v : hp) o = std::max(o, v);
+    for (int b = 0; b < NRED; ++b) o = std::max(o, hp[b]);
     if (std::isinf(o) || !(o == o)) throw Error(ST_NONFINITE, "svd: non-finite matrix entries");
     return o;
   };
@@ -1076,9 +1076,8 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
       // the deviation delta = |Q^H Q - I|_2 to (3/4) delta^2 + O(delta^3),
       // with delta_0 <= b: the Frobenius norm of Q^H Q - I is read before
       // steps 2 and 4, and the step after a reading <= 1e-8 is the last.
-      std::vector<double>& cp = col_host;
-      cp.resize(2 * (size_t)Np);
-      DMRG_HIP(hipMemcpyAsync(cp.data(), w.colpart, 2 * (size_t)Np * sizeof(double),
+      double* const cp = w.host;  // 2 Np column sums (host capacity 2 cap + 2 NRED)
+      DMRG_HIP(hipMemcpyAsync(cp, w.colpart, 2 * (size_t)Np * sizeof(double),
                               hipMemcpyDeviceToHost, st));
       DMRG_HIP(hipStreamSynchronize(st));
       double e2 = 0.0, c1 = 0.0;
@@ -1126,11 +1125,11 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
           gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), Z, Np, P, Np, T(0), Zn, Np, tl);
           std::swap(Z, Zn);
           if (check) {
-            double dp[NRED];
-            DMRG_HIP(hipMemcpyAsync(dp, nspart, sizeof(dp), hipMemcpyDeviceToHost, st));
+            double* const dp = w.host;
+            DMRG_HIP(hipMemcpyAsync(dp, nspart, NRED * sizeof(double), hipMemcpyDeviceToHost, st));
             DMRG_HIP(hipStreamSynchronize(st));
             double d2 = 0.0;
-            for (double v : dp) d2 += v;
+            for (int b = 0; b < NRED; ++b) d2 += dp[b];
             if (!(d2 == d2)) break;
             unitary = d2 <= 1e-16;  // |Q^H Q - I|_F <= 1e-8 before this step
           }
@@ -1228,6 +1227,11 @@ void SvdWork<T>::alloc(int cap_n) {
   DMRG_HIP(hipMalloc(&resid, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&scal, 4 * sizeof(double)));
   DMRG_HIP(hipMalloc(&newk, sizeof(int)));
+  // every device-to-host read of the solver lands in pinned memory, so no
+  // copy goes through the runtime's pageable staging path
+  DMRG_HIP(hipHostMalloc((void**)&host, (2 * (size_t)cap + 2 * NRED + 16) * sizeof(double),
+                         hipHostMallocDefault));
+  DMRG_HIP(hipHostMalloc((void**)&host_i, 4 * sizeof(int), hipHostMallocDefault));
 }
 
 template <class T>
@@ -1240,6 +1244,10 @@ void SvdWork<T>::release() {
   void* ps[] = {Y, W, Q, qflag, perm, rank, gdiag, red, colpart, sigma, resid, scal, newk};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  if (host) (void)hipHostFree(host);
+  if (host_i) (void)hipHostFree(host_i);
+  host = nullptr;
+  host_i = nullptr;
   Y = nullptr;
   W = nullptr;
   Q = nullptr;
@@ -1361,11 +1369,11 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
   hipLaunchKernelGGL(k_rank, dim3(1), dim3(256), 0, st, (const double*)w.sigma,
                      (const double*)w.gdiag, k, cutoff, rel_cutoff, w.scal, w.newk);
   DMRG_LAUNCH_CHECK();
-  int nk = 0;
-  double disc = 0.0;
-  DMRG_HIP(hipMemcpyAsync(&nk, w.newk, sizeof(int), hipMemcpyDeviceToHost, st));
-  DMRG_HIP(hipMemcpyAsync(&disc, w.scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  DMRG_HIP(hipMemcpyAsync(w.host_i, w.newk, sizeof(int), hipMemcpyDeviceToHost, st));
+  DMRG_HIP(hipMemcpyAsync(w.host, w.scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
   DMRG_HIP(hipStreamSynchronize(st));
+  const int nk = w.host_i[0];
+  const double disc = w.host[0];
   if (keep > 0 && nk < k) {
     // the cutoff cuts inside the kept group, whose vectors are not resolved:
     // redo the split with every pair converged

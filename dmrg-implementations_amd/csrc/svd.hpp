@@ -53,6 +53,8 @@ struct SvdWork {
   double* resid = nullptr;       // first-pass QR residuals
   double* scal = nullptr;        // [0] |rho|_F, [1] discarded weight, [2] trace(rho)
   int* newk = nullptr;
+  double* host = nullptr;  // pinned host staging of every read-back (2 cap + 2*256 + 16 doubles)
+  int* host_i = nullptr;   // pinned host staging of the kept rank
   int cap = 0;  // largest supported rho dimension (multiple of 32)
   void alloc(int cap_n);
   void release();

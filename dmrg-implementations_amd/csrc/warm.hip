@@ -70,7 +70,7 @@ void overlap_right_step(hipStream_t st, const T* Tin, int r, int c, const T* Bol
 
 template <class T>
 bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
-                   double start_tol, T** Zout) {
+                   double* hp, double start_tol, T** Zout) {
   const bool cx = is_cplx<T>::value;
   DMRG_HIP(hipMemcpyAsync(Z0, Tin, (size_t)n * n * sizeof(T), hipMemcpyDeviceToDevice, st));
   T* Z = Z0;
@@ -80,8 +80,6 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
   // l = 3e-4 the scaled iteration needs about 13; steps past convergence keep
   // Z unitary to rounding, the iteration being stable at its fixed point, and
   // by then a = 1: the bound l reaches 1 no later than the singular values).
-  static thread_local std::vector<double> hp;
-  hp.resize((size_t)kPolarMaxIter * kPolarParts);
   int seen = 0;
   double l = kPolarL0;  // lower bound of the smallest singular value of Z
   for (int it = 0; it < kPolarMaxIter; ++it) {
@@ -92,7 +90,7 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
                        dev + (size_t)it * kPolarParts);
     DMRG_LAUNCH_CHECK();
     if (it == 0 || (it >= 9 && it % 2 == 1) || it == kPolarMaxIter - 1) {
-      DMRG_HIP(hipMemcpyAsync(hp.data() + (size_t)seen * kPolarParts, dev + (size_t)seen * kPolarParts,
+      DMRG_HIP(hipMemcpyAsync(hp + (size_t)seen * kPolarParts, dev + (size_t)seen * kPolarParts,
                               (size_t)(it + 1 - seen) * kPolarParts * sizeof(double),
                               hipMemcpyDeviceToHost, st));
       DMRG_HIP(hipStreamSynchronize(st));
@@ -142,7 +140,8 @@ void warm_basis_right(hipStream_t st, const T* Z, int cR, int d, const T* X, T* 
                                      int, int, T*, T*);                                         \
   template void overlap_right_step<T>(hipStream_t, const T*, int, int, const T*, int, const T*, \
                                       int, int, T*, T*);                                        \
-  template bool polar_unitary<T>(hipStream_t, const T*, int, T*, T*, T*, double*, double, T**);  \
+  template bool polar_unitary<T>(hipStream_t, const T*, int, T*, T*, T*, double*, double*,      \
+                                 double, T**);                                                  \
   template void warm_basis_left<T>(hipStream_t, const T*, int, int, const T*, T*);               \
   template void warm_basis_right<T>(hipStream_t, const T*, int, int, const T*, T*);
 DMRG_WARM_INST(double)

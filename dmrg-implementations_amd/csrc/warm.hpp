@@ -46,11 +46,12 @@ constexpr double kPolarL0 = 3e-4;  // initial lower bound of the smallest singul
 // max|T^H T - I| > start_tol, when the iteration stalls (a singular value
 // near 0) or when it does not reach max|Z^H Z - I| <= 1e-14 in kPolarMaxIter
 // steps.  Z0, Z1, P are n x n work buffers; dev holds kPolarMaxIter *
-// kPolarParts device doubles.  Synchronises the stream after the first step,
-// then after steps 9, 11, 13, ...
+// kPolarParts device doubles and hp as many pinned host doubles (the
+// read-backs of the deviation).  Synchronises the stream after the first
+// step, then after steps 9, 11, 13, ...
 template <class T>
 bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
-                   double start_tol, T** Zout);
+                   double* hp, double start_tol, T** Zout);
 
 // Left side: out[(a + cL s), j] = sum_a' conj(Z[a', a]) X[(a' + cL s), j],
 // N = cL d columns (ld N).

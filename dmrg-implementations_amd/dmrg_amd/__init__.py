@@ -33,7 +33,7 @@ EXPORTED = [
     "dmrg_init_mps_random", "dmrg_init_mps_product", "dmrg_init_mps_neel", "dmrg_set_mps",
     "dmrg_run", "dmrg_run_batch", "dmrg_get_energy", "dmrg_get_sweep_energies", "dmrg_get_bond_dims",
     "dmrg_get_mps", "dmrg_get_env", "dmrg_expectation", "dmrg_get_stats", "dmrg_reset_stats", "dmrg_last_error",
-    "dmrg_ctx_create", "dmrg_ctx_destroy", "dmrg_op_heff2", "dmrg_op_heff1", "dmrg_op_env_left",
+    "dmrg_ctx_create", "dmrg_ctx_destroy", "dmrg_op_heff2", "dmrg_op_heff2_f32", "dmrg_op_heff1", "dmrg_op_env_left",
     "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
     "dmrg_op_gemm", "dmrg_op_gemm_f32", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h", "dmrg_memcpy_d2d",
     "dmrg_dev_synchronize", "dmrg_device_count",
@@ -104,6 +104,7 @@ def lib():
         "dmrg_ctx_create": (i, [i, C.POINTER(vp)]),
         "dmrg_ctx_destroy": (None, [vp]),
         "dmrg_op_heff2": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp, vp]),
+        "dmrg_op_heff2_f32": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp, vp]),
         "dmrg_op_heff1": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp]),
         "dmrg_op_env_left": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp]),
         "dmrg_op_env_right": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp]),
@@ -394,6 +395,17 @@ class Ops:
         cL, cR, d, D = self._two_site(L, R, W1, W2, theta, out)
         _check(lib().dmrg_op_heff2(self._h, stream, self._dt(theta), cL, cR, d, D, L.ptr, R.ptr,
                                    W1.ptr, W2.ptr, theta.ptr, out.ptr))
+
+    def heff2_f32(self, L, R, W1, W2, theta, out, prec=PREC_FP32, stream=None):
+        """Mixed-mode two-site matvec: float32 L, R, theta, out; float64 W1, W2
+        (fused on the host, rounded to FP32); FP32 MFMA or split-BF16 (prec)."""
+        cL, D = L.shape[0], L.shape[1]
+        cR, d = R.shape[0], theta.shape[1]
+        self._expect(np.float32, L=(L, (cL, D, cL)), R=(R, (cR, D, cR)),
+                     theta=(theta, (cL, d, d, cR)), out=(out, (cL, d, d, cR)))
+        self._expect(np.float64, W1=(W1, (D, d, d, D)), W2=(W2, (D, d, d, D)))
+        _check(lib().dmrg_op_heff2_f32(self._h, stream, prec, cL, cR, d, D, L.ptr, R.ptr, W1.ptr,
+                                       W2.ptr, theta.ptr, out.ptr))
 
     def heff1(self, L, R, W, theta, out, stream=None):
         cL, D = L.shape[0], L.shape[1]

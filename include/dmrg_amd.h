@@ -206,6 +206,15 @@ void dmrg_ctx_destroy(dmrg_ctx_handle c);
 int dmrg_op_heff2(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, int d, int D,
                   const void* L, const void* R, const void* W1, const void* W2,
                   const void* theta, void* out);
+/* The two-site matvec of the mixed modes: L, R, theta, out in FP32 (layouts
+ * as dmrg_op_heff2), W1/W2 the raw FP64 MPO tensors (device), fused on the
+ * host and rounded to FP32; prec DMRG_PREC_FP32 (FP32 MFMA) or
+ * DMRG_PREC_BF16X3 (split-BF16 products).  The reference has no FP32 path:
+ * this is the mixed modes' substitute for apply_heff_two_site
+ * (dmrg2_gpu_impl.h:606-780). */
+int dmrg_op_heff2_f32(dmrg_ctx_handle c, void* stream, int prec, int cL, int cR, int d, int D,
+                      const float* L, const float* R, const double* W1, const double* W2,
+                      const float* theta, float* out);
 /* Single-site matvec: DMRGGPU::apply_heff (dmrg_gpu_impl.h:538-705). */
 int dmrg_op_heff1(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, int d, int D,
                   const void* L, const void* R, const void* W, const void* theta, void* out);
