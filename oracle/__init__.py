@@ -20,7 +20,9 @@ dmrg        two-site / single-site sweep restated from the reference GPU
             sweep order, returned energy).
 pdmrg_serial  the reference CPU path (tensordot H_eff + ARPACK eigsh +
             numpy SVD), used as the timed CPU baseline.
-binio       reader/writer for the reference's ``.bin`` MPS/MPO format.
+
+The reader/writer of the reference's ``.bin`` MPS/MPO format is product code
+(``dmrg_amd/io.py``), not part of the oracle.
 
 Pinning status of each function is listed in the module docstrings; every
 function here is checked in ``tests/test_oracle_*.py`` against the

@@ -164,8 +164,19 @@ def _graded(rng, m, n, cplx, decay):
 def test_svd_split_at_config_shape(dmrg_amd, ops, scale_left, m, k, cplx, decay):
     """The truncated split of the C4 (1024^2, keep 512) and C3 (1280^2
     complex, keep 256) bonds against LAPACK on a graded spectrum: kept
-    singular values to 1e-12 sigma_max, the rank-k product equal to LAPACK's
-    best rank-k approximation, and the orthonormal factor orthonormal."""
+    singular values to 1e-12 sigma_max, the truncation residual equal to the
+    optimal one, the rank-k product equal to LAPACK's best rank-k
+    approximation within the split's accuracy class, and the orthonormal
+    factor orthonormal.
+
+    The split diagonalises the Gram matrix rho = M M^H by Jacobi until every
+    off-diagonal entry is below tau = 1e-14 ||rho||_F.  A coupling tau
+    between the last kept and the first discarded direction turns the kept
+    space by at most tau / (s_{k-1}^2 - s_k^2), which moves the rank-k product
+    by that angle times ~s_{k-1}.  At the C4 cut (s = e^{-j/60}, k = 512) the
+    Gram gap is 1.3e-9, so the product may differ from LAPACK's (which
+    resolves the gap s_{k-1} - s_k directly) by ~1e-9 relative while its
+    residual stays optimal to second order in the angle."""
     rng = np.random.default_rng(m + k + (1 if scale_left else 0))
     M, s = _graded(rng, m, m, cplx, decay)
     left, right, sig, nk = ops.svd_split(_dev(dmrg_amd, M), k, scale_left=scale_left)
@@ -173,7 +184,10 @@ def test_svd_split_at_config_shape(dmrg_amd, ops, scale_left, m, k, cplx, decay)
     assert np.max(np.abs(sig - s[:k])) < 1e-12 * s[0]
     U, S, Vh = np.linalg.svd(M)
     best = (U[:, :k] * S[:k]) @ Vh[:k]
-    assert rel_err(left @ right, best) < 1e-10
+    tau = 1e-14 * np.sqrt(np.sum(S ** 4))
+    angle = tau / (S[k - 1] ** 2 - S[k] ** 2)
+    bound = 4 * angle * S[k - 1] / np.linalg.norm(best) + 1e-12
+    assert rel_err(left @ right, best) < bound, bound
     resid = np.linalg.norm(M - left @ right)
     assert abs(resid - np.sqrt(np.sum(S[k:] ** 2))) < 1e-10 * np.linalg.norm(M)
     q = left if not scale_left else right.conj().T
@@ -260,16 +274,21 @@ def test_c4_mixed_matches_fp64(dmrg_amd, c4_fp64, prec):
     assert abs(n2 - 1) < 1e-9 and abs(e - E) < 1e-9 * wl["L"]
 
 
+C5_SWEEPS = 6
+
+
 @pytest.fixture(scope="module")
 def c5_sequential(dmrg_amd):
-    """Each of the 64 C5 chains solved alone in plain FP64 (4 sweeps)."""
+    """Each of the 64 C5 chains solved alone in plain FP64 (6 sweeps, the
+    benchmark's count: after 4 the larger chains, e.g. L=32 chi=256, are
+    still moving by ~1e-9 relative between sweeps)."""
     chains = bench.batch_chains(bench.WORKLOADS["c5"])
     ref = []
     for i, (L, chi, W, h) in enumerate(chains):
         e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20)
         e.set_mpo(bench.heisenberg_mpo(L, h))
         e.initialize_mps_random(seed=1000 + i, scale=0.1)
-        ref.append(e.run(4))
+        ref.append(e.run(C5_SWEEPS))
         e.close()
     return chains, np.array(ref)
 
@@ -286,7 +305,7 @@ def test_c5_batch_matches_sequential(dmrg_amd, c5_sequential, prec):
         e.set_mpo(bench.heisenberg_mpo(L, h))
         e.initialize_mps_random(seed=1000 + i, scale=0.1)
         engines.append(e)
-    E, T = dmrg_amd.run_batch(engines, 4, threads=8)
+    E, T = dmrg_amd.run_batch(engines, C5_SWEEPS, threads=8)
     for e in engines:
         e.close()
     rel = np.abs(E - ref) / np.abs(ref)
