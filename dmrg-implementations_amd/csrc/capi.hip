@@ -518,6 +518,14 @@ int dmrg_op_env_left(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co,
   });
 }
 
+int dmrg_op_env_left_noalloc(void* stream, int dtype, int ci, int co, int d, int D, const void* L,
+                             const void* A, const void* W, void* Lnew) {
+  if (!L || !A || !W || !Lnew) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
+  return guard([&] {
+    env_left_noalloc_op((hipStream_t)stream, dtype, ci, co, d, D, L, A, W, Lnew);
+  });
+}
+
 int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co, int d, int D,
                       const void* R, const void* A, const void* W, void* Rnew) {
   if (!c || !R || !A || !W || !Rnew) return fail(DMRG_ERR_BAD_ARG, "NULL argument");

@@ -70,4 +70,10 @@ void convert_f32_f64_scaled(hipStream_t st, const float* in, double* out, size_t
 template <class T>
 void set_boundary_env(hipStream_t st, T* env, int D, int w);  // (1, D, 1) unit vector e_w
 
+// env_left on the library's own fixed device scratch (noalloc.hip): no
+// allocation, no context.  dtype is a DMRG_DTYPE_* code.  Throws BAD_ARG
+// when the two intermediates exceed DMRG_NOALLOC_SCRATCH_BYTES.
+void env_left_noalloc_op(hipStream_t st, int dtype, int ci, int co, int d, int D, const void* L,
+                         const void* A, const void* W, void* Lnew);
+
 }  // namespace dmrg

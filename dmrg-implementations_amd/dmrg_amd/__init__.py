@@ -34,7 +34,7 @@ EXPORTED = [
     "dmrg_run", "dmrg_run_batch", "dmrg_get_energy", "dmrg_get_sweep_energies", "dmrg_get_bond_dims",
     "dmrg_get_mps", "dmrg_get_env", "dmrg_expectation", "dmrg_get_stats", "dmrg_reset_stats", "dmrg_last_error",
     "dmrg_ctx_create", "dmrg_ctx_destroy", "dmrg_op_heff2", "dmrg_op_heff2_f32", "dmrg_op_heff1", "dmrg_op_env_left",
-    "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
+    "dmrg_op_env_left_noalloc", "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
     "dmrg_op_gemm", "dmrg_op_gemm_f32", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h", "dmrg_memcpy_d2d",
     "dmrg_dev_synchronize", "dmrg_device_count",
 ]
@@ -107,6 +107,7 @@ def lib():
         "dmrg_op_heff2_f32": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp, vp]),
         "dmrg_op_heff1": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp]),
         "dmrg_op_env_left": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp]),
+        "dmrg_op_env_left_noalloc": (i, [vp, i, i, i, i, i, vp, vp, vp, vp]),
         "dmrg_op_env_right": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp]),
         "dmrg_op_svd_split": (i, [vp, vp, i, i, i, i, d, d, i, vp, vp, vp, vp, C.POINTER(i)]),
         "dmrg_op_heig": (i, [vp, vp, i, i, vp, vp, vp, C.POINTER(i)]),
@@ -518,6 +519,23 @@ def run_batch(engines, n_sweeps, threads=8):
     T = _np.zeros(max(n, 1))
     _check(lib().dmrg_run_batch(hs, n, n_sweeps, threads, E.ctypes.data, T.ctypes.data))
     return E[:n], T[:n]
+
+
+# bytes of the library's fixed device scratch behind env_left_noalloc
+NOALLOC_SCRATCH_BYTES = 16 << 20
+
+
+def env_left_noalloc(L, A, W, Lnew, stream=None):
+    """dmrg_op_env_left_noalloc: the left-environment update of
+    Ops.env_left on the library's fixed device scratch (no allocation, no
+    context), on the current device.  Calls that overlap on one GPU share
+    that scratch and race."""
+    ci, d, co = A.shape
+    D = W.shape[0]
+    Ops._expect(A.dtype, L=(L, (ci, D, ci)), A=(A, (ci, d, co)), W=(W, (D, d, d, D)),
+                Lnew=(Lnew, (co, D, co)))
+    _check(lib().dmrg_op_env_left_noalloc(stream, dtype_code(A.dtype), ci, co, d, D, L.ptr, A.ptr,
+                                          W.ptr, Lnew.ptr))
 
 
 def copy_device(dst, src):

@@ -221,6 +221,22 @@ int dmrg_op_heff1(dmrg_ctx_handle c, void* stream, int dtype, int cL, int cR, in
 /* update_left_env (dmrg2_gpu_impl.h:786-860): Lnew (co,D,co). */
 int dmrg_op_env_left(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co, int d, int D,
                      const void* L, const void* A, const void* W, void* Lnew);
+/* Size of the fixed device scratch behind dmrg_op_env_left_noalloc: 16 MiB
+ * per GPU, held from the first call on that GPU until the process ends. */
+#define DMRG_NOALLOC_SCRATCH_BYTES (16u << 20)
+/* dmrg_op_env_left without allocation.  No reference counterpart: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and result as dmrg_op_env_left, but no context: the two
+ * intermediates (D*d*ci*co scalars each) live in the library's fixed device
+ * scratch, so it returns BAD_ARG when 2*D*d*ci*co*sizeof(scalar) (each
+ * rounded up to 256 bytes) exceeds DMRG_NOALLOC_SCRATCH_BYTES (e.g. FP64
+ * ci = co = 256, d = 2, D = 5 fits).  Runs on the calling thread's current
+ * device.  All calls on one GPU share that scratch: calls that overlap on
+ * one GPU (two streams or two threads) race; calls in order on one stream
+ * do not. */
+int dmrg_op_env_left_noalloc(void* stream, int dtype, int ci, int co, int d, int D,
+                             const void* L, const void* A, const void* W, void* Lnew);
 /* update_right_env (dmrg2_gpu_impl.h:866-934): A (co,d,ci), R (ci,D,ci). */
 int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co, int d, int D,
                       const void* R, const void* A, const void* W, void* Rnew);
