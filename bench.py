@@ -40,7 +40,10 @@ sys.path.insert(0, os.path.join(ROOT, "dmrg-implementations_amd"))
 sys.path.insert(0, ROOT)
 
 # MI355X dense MFMA TFLOP/s; split-BF16 counts the BF16 peak over its six partial products
-PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 6}
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 6, "fp32env": 157.3}
+# mode -> (precision, environment precision); fp32env is the FP32 matvec with
+# the environment updates on FP32 MFMA too (FP32 environments)
+MODE = {"fp64": (0, 0), "fp32": (1, 0), "bf16x3": (2, 0), "fp32env": (1, 1)}
 
 WORKLOADS = {
     # the headline: Heisenberg XXX L=128 chi=512, FP64 vs mixed
@@ -124,14 +127,16 @@ def workload_mpo(wl):
 # (whose two-site vectors carry FP32 rounding; measured at C4: mixed split
 # 4.1 s instead of 5.2 s per 3 sweeps, |dE|/|E| 8e-12 against FP64; 1e-12
 # gave 1.3e-10).
-SVD_TOL = {"fp64": 1e-14, "fp32": 1e-13, "bf16x3": 1e-13}
+SVD_TOL = {"fp64": 1e-14, "fp32": 1e-13, "bf16x3": 1e-13, "fp32env": 1e-13}
 
 
-def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
+def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol, env_precision=0, rsvd=-1):
     Ws = workload_mpo(wl)
     eng = dmrg_amd.DMRG(wl["L"], wl.get("d", 2), wl["chi"], wl.get("D", 5),
                         dtype=wl.get("dtype", "f64"), tol=0.0, krylov_max=wl["krylov"],
-                        precision=precision, svd_tol=svd_tol, svd_tol_mixed=svd_tol, profile=1)
+                        precision=precision, svd_tol=svd_tol, svd_tol_mixed=svd_tol, profile=1,
+                        env_precision=env_precision, svd_rsvd=1 if rsvd >= 0 else 0,
+                        rsvd_power=max(rsvd, 0), autotune=1)
     eng.set_mpo(Ws)
     eng.initialize_mps_random(seed=wl["seed"], scale=0.1)
     warm_sweeps = []
@@ -162,8 +167,26 @@ def run_gpu(dmrg_amd, wl, precision, steps, warmup, svd_tol):
                sweep_s_median=sorted(per_sweep)[len(per_sweep) // 2] if per_sweep else 0.0,
                warmup_sweep_s=warm_sweeps,
                sweep_s_mean_all=(sum(warm_sweeps) + sum(per_sweep)) / max(len(warm_sweeps) + len(per_sweep), 1),
-               ms_per_matvec=st["heff_ms"] / max(st["matvecs"], 1))
+               ms_per_matvec=st["heff_ms"] / max(st["matvecs"], 1),
+               env_ms=st["env_ms"], env_flop=st["env_flop"],
+               env_tflops=st["env_flop"] / (st["env_ms"] * 1e-3) / 1e12 if st["env_ms"] > 0 else 0.0,
+               trunc_err_last_sweep=st["trunc_sum_last"], max_discarded=st["max_discarded"],
+               env_precision="fp32" if (precision != 0 and eng.opts.env_precision) else "fp64")
     return eng, res
+
+
+def sweep_bond_equivalents(wl, c0, nbonds):
+    """A sweep's 2(L-1) bond updates in units of the mean of the measured
+    bonds c0 .. c0+nbonds-1, weighting each bond by its H_eff flops
+    F_heff2(cL, cR) at the pre-sized bond dimensions min(d^min(i, L-i), chi)
+    (the H_eff dominates the CPU bond step)."""
+    L, chi = wl["L"], wl["chi"]
+    d, D = wl.get("d", 2), wl.get("D", 5)
+    bonds = [min(d ** min(i, L - i), chi) for i in range(L + 1)]
+    f = lambda cl, cr: 2.0 * D * d * d * cl * cr * (cl + cr) + 2.0 * cl * cr * (D * d * d) ** 2
+    w = [f(bonds[i], bonds[i + 2]) for i in range(L - 1)]
+    w_meas = sum(w[c0:c0 + nbonds]) / nbonds
+    return 2 * sum(w) / w_meas
 
 
 def cpu_baseline(eng, wl, nbonds):
@@ -185,10 +208,12 @@ def cpu_baseline(eng, wl, nbonds):
         ps.bond_step(A, Ws, L_envs, R_envs, i, "right", chi, max_iter=wl["krylov"], tol=1e-10)
     t = time.perf_counter() - t0
     per_bond = t / nbonds
+    bond_equiv = sweep_bond_equivalents(wl, c0, nbonds)
     cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
-    return dict(value=per_bond * 2 * (L - 1), unit="s/sweep", cores=cores, kind="port",
+    return dict(value=per_bond * bond_equiv, unit="s/sweep", cores=cores, kind="port",
                 sample="%d centre bonds (L->R, chi=%d) of the same chain and state, %.2f s/bond, "
-                       "scaled by the 2*(L-1) bond updates of a sweep" % (nbonds, chi, per_bond))
+                       "scaled to a sweep by the H_eff flops of its 2*(L-1) bond updates "
+                       "(%.1f centre-bond equivalents)" % (nbonds, chi, per_bond, bond_equiv))
 
 
 def heff_traffic(workload, mode):
@@ -228,7 +253,7 @@ def run_batch_bench(dmrg_amd, wl, a):
     sweeps = a.batch_sweeps
     ref, t_seq = [], 0.0
     for i, (L, chi, W, h) in enumerate(chains):
-        e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=wl["krylov"])
+        e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=wl["krylov"], autotune=1)
         e.set_mpo(heisenberg_mpo(L, h))
         e.initialize_mps_random(seed=1000 + i, scale=0.1)
         dmrg_amd.synchronize()
@@ -243,7 +268,8 @@ def run_batch_bench(dmrg_amd, wl, a):
     for m, prec in (("fp64", dmrg_amd.PREC_FP64), ("fp32", dmrg_amd.PREC_FP32)):
         engines = []
         for i, (L, chi, W, h) in enumerate(chains):
-            e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=wl["krylov"], precision=prec)
+            e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=wl["krylov"], precision=prec,
+                              graph=a.graph, autotune=1)
             e.set_mpo(heisenberg_mpo(L, h))
             e.initialize_mps_random(seed=1000 + i, scale=0.1)
             engines.append(e)
@@ -268,7 +294,9 @@ def run_batch_bench(dmrg_amd, wl, a):
         "data": "synthetic (seeded random-field chains and initial MPS)",
         "config": {"workload": "64 random-field Heisenberg chains, L in {16,24,32,48,64}, chi_max in "
                                "{32,64,128,256}, W in {0.5,1,2,5}, seed %d, %d sweeps each, %d host "
-                               "threads" % (wl["seed"], sweeps, a.batch_threads)},
+                               "threads, Lanczos graphs %s" % (wl["seed"], sweeps, a.batch_threads,
+                                                               "on" if a.graph else "off"),
+                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")},
         "sequential_fp64_s": t_seq, "modes": out_modes}), flush=True)
 
 
@@ -285,6 +313,17 @@ def main():
     ap.add_argument("--modes", default="fp64,fp32,bf16x3")
     ap.add_argument("--batch-sweeps", type=int, default=6, help="c5: sweeps per chain")
     ap.add_argument("--batch-threads", type=int, default=8, help="c5: host threads (chains in flight)")
+    ap.add_argument("--rsvd", type=int, default=-1, metavar="POWER",
+                    help="randomised split with POWER subspace iterations on the bonds where the "
+                         "reference uses it (-1: exact split everywhere)")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="c5: batched chains replay their Lanczos stretches as HIP graphs (1) "
+                         "or launch every kernel (0)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="set GPU_MAX_HW_QUEUES (hardware queues of this process, at most 32) "
+                         "before the runtime starts; 0 leaves the environment as it is.  The "
+                         "batch runner's chains run on their own streams, which share these "
+                         "queues")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="before the runs, copy MIB MiB device-to-device 3 times (the known-size "
                          "copy kernel that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -298,6 +337,10 @@ def main():
         print("bench.py: measures one GPU only; the sweep of a single chain is sequential over its "
               "bonds and is not sharded across GPUs.  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if a.hw_queues:
+        if not 1 <= a.hw_queues <= 32:
+            ap.error("--hw-queues must be in [1, 32]")
+        os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     import dmrg_amd
     wl = WORKLOADS[a.workload]
     if a.calib_copy > 0:
@@ -317,11 +360,11 @@ def main():
         modes = [m for m in modes if m == "fp64"] or ["fp64"]  # the FP32 matvec is real-only
     res, engines = {}, {}
     for m in modes:
-        prec = {"fp64": dmrg_amd.PREC_FP64, "fp32": dmrg_amd.PREC_FP32,
-                "bf16x3": dmrg_amd.PREC_BF16X3}[m]
+        prec, envp = MODE[m]
         print("[bench] %s: %d warm-up + %d timed sweeps, L=%d chi=%d" %
               (m, a.warmup, a.steps, wl["L"], wl["chi"]), file=sys.stderr, flush=True)
-        eng, r = run_gpu(dmrg_amd, wl, prec, a.steps, a.warmup, SVD_TOL[m])
+        eng, r = run_gpu(dmrg_amd, wl, prec, a.steps, a.warmup, SVD_TOL[m], envp, a.rsvd)
+        r["rsvd_calls"] = eng.stats()["rsvd_calls"]
         r["svd_tol"] = SVD_TOL[m]
         res[m], engines[m] = r, eng
         print("[bench] %s: %s" % (m, json.dumps(r)), file=sys.stderr, flush=True)
@@ -371,10 +414,16 @@ def main():
         out["dE_abs"] = abs(e32 - e64)
         out["dE_rel"] = abs(e32 - e64) / abs(e64)
         out["speedup_mixed_vs_fp64"] = res["fp64"]["sweep_s"] / res["fp32"]["sweep_s"]
+    # truncation error (discarded weight summed over the last sweep's splits)
+    # of every mode, next to the FP64 run's
+    out["trunc_err_last_sweep"] = {m: res[m]["trunc_err_last_sweep"] for m in res}
     for m in res:
         if m != "fp64" and "fp64" in res:
             res[m]["dE_rel_vs_fp64"] = abs(res[m]["energy"] - res["fp64"]["energy"]) / abs(res["fp64"]["energy"])
             res[m]["speedup_vs_fp64"] = res["fp64"]["sweep_s"] / res[m]["sweep_s"]
+            t64 = res["fp64"]["trunc_err_last_sweep"]
+            res[m]["trunc_err_rel_vs_fp64"] = (abs(res[m]["trunc_err_last_sweep"] - t64) / t64
+                                               if t64 > 0 else None)
         res[m]["heff_frac_of_peak"] = res[m]["heff_tflops"] / PEAK[m]
     print(json.dumps(out), flush=True)
 

@@ -19,7 +19,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "dmrg_amd", "libdmrg_amd.so")
 BIN = os.path.join(HERE, "bin", "dmrg2_gpu_amd")
 
-SOURCES = ["gemm.hip", "contract.hip", "lanczos.hip", "svd.hip", "warm.hip", "engine.hip", "noalloc.hip", "capi.hip"]
+SOURCES = ["gemm.hip", "contract.hip", "lanczos.hip", "svd.hip", "warm.hip", "engine.hip", "pdmrg.hip", "noalloc.hip",
+           "capi.hip"]
 CLI = ["cli.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -57,6 +57,12 @@ Options to_opts(const dmrg_options* o) {
     r.fp32_eig_rel = o->fp32_eig_rel;
     r.svd_subspace = o->svd_subspace;
     r.svd_tol_mixed = o->svd_tol_mixed;
+    r.env_precision = o->env_precision;
+    r.svd_rsvd = o->svd_rsvd;
+    r.rsvd_oversample = o->rsvd_oversample;
+    r.rsvd_power = o->rsvd_power;
+    r.graph = o->graph;
+    r.autotune = o->autotune;
   }
   return r;
 }
@@ -93,6 +99,12 @@ void dmrg_default_options(dmrg_options* o) {
   o->fp32_eig_rel = d.fp32_eig_rel;
   o->svd_subspace = d.svd_subspace;
   o->svd_tol_mixed = d.svd_tol_mixed;
+  o->env_precision = d.env_precision;
+  o->svd_rsvd = d.svd_rsvd;
+  o->rsvd_oversample = d.rsvd_oversample;
+  o->rsvd_power = d.rsvd_power;
+  o->graph = d.graph;
+  o->autotune = d.autotune;
 }
 
 const char* dmrg_last_error(void) { return g_err.c_str(); }
@@ -217,6 +229,22 @@ int dmrg_run(dmrg_handle h, int n_sweeps, int quiet, double* energy) {
   });
 }
 
+int dmrg_run_pdmrg(dmrg_handle h, int segments, int outer, int local, int warmup, int polish,
+                   int quiet, double* energy) {
+  if (!h) return fail(DMRG_ERR_BAD_ARG, "NULL handle");
+  return guard([&] {
+    PdmrgParams p;
+    p.segments = segments;
+    p.outer = outer;
+    p.local = local;
+    p.warmup = warmup;
+    p.polish = polish;
+    double e = 0.0;
+    h->with([&](auto& eng) { e = eng.run_pdmrg(p, quiet != 0); });
+    if (energy) *energy = e;
+  });
+}
+
 int dmrg_get_energy(dmrg_handle h, double* energy) {
   if (!h || !energy) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
   return guard([&] { h->with([&](auto& e) { *energy = e.energy; }); });
@@ -281,6 +309,11 @@ int dmrg_get_stats(dmrg_handle h, dmrg_stats* st) {
       st->svd_warm_starts = s.svd_warm_starts;
       st->svd_bip_sweeps = s.svd_bip_sweeps;
       st->svd_oa_steps = s.svd_oa_steps;
+      st->trunc_sum_last = s.trunc_sum_last;
+      st->rsvd_calls = s.rsvd_calls;
+      st->graph_launches = s.graph_launches;
+      st->graph_captures = s.graph_captures;
+      st->tuned_shapes = gemm_autotune_entries();
     });
   });
 }
@@ -300,6 +333,7 @@ template <class T>
 struct CtxT {
   ContractScratch<T> sc;
   SvdWork<T> svd;
+  RsvdWork<T> rsvd;
   LzDev lz;
   T* buf = nullptr;  // derived MPO matrices
   size_t buf_cap = 0;
@@ -315,6 +349,7 @@ struct CtxT {
     if (work) (void)hipFree(work);
     for (auto e : runner.ev) (void)hipEventDestroy(e);
     svd.release();
+    rsvd.release();
     lz.release();
   }
   void need_sc(size_t n) {
@@ -567,6 +602,31 @@ int dmrg_op_svd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, 
                                  scale_left != 0, (T*)left, m, (T*)right, sigma, SvdParams{});
       DMRG_HIP(hipStreamSynchronize(st));
       *nk_out = r.nk;
+    });
+  });
+}
+
+int dmrg_op_rsvd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, int k,
+                       int oversample, int power, uint64_t seed, double cutoff,
+                       double rel_cutoff, int scale_left, const void* M, void* left,
+                       void* right, int* nk_out) {
+  if (!c || !M || !left || !right || !nk_out) return fail(DMRG_ERR_BAD_ARG, "NULL argument");
+  return guard([&] {
+    use(c);
+    DMRG_REQUIRE(m >= 1 && n >= 1 && k >= 1 && k <= std::min(m, n), "bad shape");
+    DMRG_REQUIRE(oversample >= 0 && power >= 0 && power <= 8, "bad oversample / power");
+    by_dtype(dtype, [&](auto tag) {
+      using T = decltype(tag);
+      auto& x = ctx_of<T>(c);
+      hipStream_t st = (hipStream_t)stream;
+      const int r = std::min(k + oversample, std::min(m, n));
+      x.need_svd(std::max(std::max(m, n), r));
+      if (x.rsvd.len < std::max(m, n) || x.rsvd.r < r) x.rsvd.alloc(std::max(m, n), r);
+      SvdResult res = rsvd_split<T>(st, x.svd, x.rsvd, (const T*)M, m, n, m, k, cutoff,
+                                    rel_cutoff, scale_left != 0, (T*)left, m, (T*)right,
+                                    SvdParams{}, oversample, power, seed);
+      DMRG_HIP(hipStreamSynchronize(st));
+      *nk_out = res.nk;
     });
   });
 }

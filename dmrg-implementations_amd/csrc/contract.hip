@@ -190,6 +190,50 @@ __global__ void k_f32_f64_scaled(const float* in, double* out, size_t n, const d
        i += (size_t)gridDim.x * blockDim.x)
     out[i] = (double)in[i] * f;
 }
+__global__ __launch_bounds__(256) void k_absmax_f32(const float* in, size_t n,
+                                                   unsigned long long* out) {
+  __shared__ float red[256];
+  const int t = threadIdx.x;
+  float m = 0.f;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + t; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(in[i]));
+  red[t] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmaxf(red[t], red[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    unsigned long long bits;
+    double v = (double)red[0];
+    memcpy(&bits, &v, sizeof(v));
+    atomicMax(out, bits);  // non-negative doubles order as their bit patterns
+  }
+}
+__global__ void k_rescale_f32(float* e, size_t n, const double* maxv, const double* s_in,
+                              double* s_out) {
+  const double m = unit_scale(*maxv);
+  const double inv = 1.0 / m;  // in double: m may be an FP32 denormal
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    e[i] = (float)((double)e[i] * inv);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *s_out = unit_scale(*s_in) * m;
+}
+__global__ void k_fill_d(double* s, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    s[i] = v;
+}
+template <class T>
+__global__ void k_copy2d(T* __restrict__ dst, size_t ldd, const T* __restrict__ src, size_t lds,
+                         size_t rows, size_t cols) {
+  const size_t tot = rows * cols;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = e % rows, j = e / rows;
+    dst[i + j * ldd] = src[i + j * lds];
+  }
+}
 template <class T>
 __global__ void k_unit(T* env, int D, int w) {
   int i = threadIdx.x;
@@ -230,6 +274,40 @@ void convert_f32_f64_scaled(hipStream_t st, const float* in, double* out, size_t
                      st, in, out, n, s1, s2);
   DMRG_LAUNCH_CHECK();
 }
+void absmax_f32(hipStream_t st, const float* in, size_t n, double* out) {
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, st, out);
+  if (n)
+    hipLaunchKernelGGL(k_absmax_f32, dim3(std::min<size_t>(cdiv(n, 256), 1024)), dim3(256), 0, st,
+                       in, n, reinterpret_cast<unsigned long long*>(out));
+  DMRG_LAUNCH_CHECK();
+}
+void rescale_env_f32(hipStream_t st, float* e, size_t n, const double* maxv, const double* s_in,
+                     double* s_out) {
+  // one block also covers n = 0 (the scale is still written)
+  hipLaunchKernelGGL(k_rescale_f32, dim3(std::max<unsigned>(1, std::min<size_t>(cdiv(n, 256), 4096))),
+                     dim3(256), 0, st, e, n, maxv, s_in, s_out);
+  DMRG_LAUNCH_CHECK();
+}
+template <class T>
+void copy_dev(hipStream_t st, T* dst, size_t ldd, const T* src, size_t lds, size_t rows,
+              size_t cols) {
+  const size_t tot = rows * cols;
+  if (!tot) return;
+  hipLaunchKernelGGL(k_copy2d<T>, dim3(std::min<size_t>(cdiv(tot, 256), 8192)), dim3(256), 0, st,
+                     dst, ldd, src, lds, rows, cols);
+  DMRG_LAUNCH_CHECK();
+}
+template void copy_dev<double>(hipStream_t, double*, size_t, const double*, size_t, size_t, size_t);
+template void copy_dev<cdouble>(hipStream_t, cdouble*, size_t, const cdouble*, size_t, size_t,
+                                size_t);
+template void copy_dev<float>(hipStream_t, float*, size_t, const float*, size_t, size_t, size_t);
+
+void fill_f64(hipStream_t st, double* s, size_t n, double v) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_d, dim3(std::min<size_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, s, n,
+                     v);
+  DMRG_LAUNCH_CHECK();
+}
 template <class T>
 void set_boundary_env(hipStream_t st, T* env, int D, int w) {
   hipLaunchKernelGGL(k_unit<T>, dim3(1), dim3(64), 0, st, env, D, w);
@@ -253,5 +331,6 @@ DMRG_INST(float)
 #undef DMRG_INST
 template void set_boundary_env<double>(hipStream_t, double*, int, int);
 template void set_boundary_env<cdouble>(hipStream_t, cdouble*, int, int);
+template void set_boundary_env<float>(hipStream_t, float*, int, int);
 
 }  // namespace dmrg

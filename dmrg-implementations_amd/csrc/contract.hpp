@@ -70,6 +70,31 @@ void convert_f32_f64_scaled(hipStream_t st, const float* in, double* out, size_t
 template <class T>
 void set_boundary_env(hipStream_t st, T* env, int D, int w);  // (1, D, 1) unit vector e_w
 
+// FP32 environments of the mixed modes: an environment is a float array e
+// with a device double s, the true environment being s * e with max |e| = 1
+// (a random start state's right environments reach ~1e67, past the FP32
+// range, so the magnitude lives in s).
+// out[0] = max |in[i]| as a double (0 for n = 0).
+void absmax_f32(hipStream_t st, const float* in, size_t n, double* out);
+// e /= m and *s_out = *s_in * m, m = *maxv (1 when 0): renormalises a freshly
+// contracted FP32 environment and carries the factor in its scale.
+void rescale_env_f32(hipStream_t st, float* e, size_t n, const double* maxv, const double* s_in,
+                     double* s_out);
+// s[0..n) = v
+void fill_f64(hipStream_t st, double* s, size_t n, double v);
+
+// dst(i, j) = src(i, j) for i < rows, j < cols (column-major, leading
+// dimensions ldd and lds) on the device, by a copy kernel: every copy of the
+// sweep is a kernel of its own rather than a runtime copy, so a stream's
+// work is kernels only.
+template <class T>
+void copy_dev(hipStream_t st, T* dst, size_t ldd, const T* src, size_t lds, size_t rows,
+              size_t cols);
+template <class T>
+inline void copy_dev(hipStream_t st, T* dst, const T* src, size_t n) {
+  copy_dev<T>(st, dst, n, src, n, n, 1);
+}
+
 // env_left on the library's own fixed device scratch (noalloc.hip): no
 // allocation, no context.  dtype is a DMRG_DTYPE_* code.  Throws BAD_ARG
 // when the two intermediates exceed DMRG_NOALLOC_SCRATCH_BYTES.

@@ -28,40 +28,95 @@ template <class T> bool finite_v(T v) { return std::isfinite(real(v)) && std::is
 // ---------------------------------------------------------------------------
 template <class T>
 double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int site,
-                             const std::function<void(const T*, T*)>& matvec) {
+                             const std::function<void(const T*, T*)>& matvec,
+                             uint64_t graph_key) {
   const int max_iter = (int)std::min<size_t>((size_t)kmax, n);
   DMRG_REQUIRE(max_iter >= 1, "lanczos: empty problem");
-  if ((int)ev.size() < 2 * max_iter) {
+  if (timed && (int)ev.size() < 2 * max_iter) {
     size_t old = ev.size();
     ev.resize(2 * max_iter);
     for (size_t i = old; i < ev.size(); ++i) DMRG_HIP(hipEventCreate(&ev[i]));
   }
-  lz_reset(st, *lz);
-  // v0 = theta/|theta|; zero-norm start replaced by the seeded stream 42+site
-  lz_norm<T>(st, *lz, theta, n);
-  lz_fill_if_small<T>(st, *lz, theta, n, (uint64_t)(42 + site), 1e-14);
-  lz_norm<T>(st, *lz, theta, n);
-  lz_scale<T>(st, *lz, V, theta, n);
-  int niter = max_iter, it = 0, nmv = 0;
-  for (it = 0; it < max_iter; ++it) {
-    T* vi = V + (size_t)it * n;
-    if (timed) DMRG_HIP(hipEventRecord(ev[2 * it], st));
-    matvec(vi, work);
-    if (timed) DMRG_HIP(hipEventRecord(ev[2 * it + 1], st));
-    ++nmv;
-    lz_alpha<T>(st, *lz, vi, work, n, it);
-    lz_sub_dots<T>(st, *lz, work, V, n, it);
-    lz_orth_norm<T>(st, *lz, work, V, n, it);
-    lz_next<T>(st, *lz, it + 1 < max_iter ? V + (size_t)(it + 1) * n : nullptr, work, n, it);
-    if (it >= 4 && it % 3 == 0) {
-      lz_check(st, *lz, it + 1, beta_tol, eig_tol, eig_rel);
+  const bool use_graph = graph_key != 0 && !timed;
+  int nmv = 0;
+  // Iterations [a, b): the prologue (v0 = theta/|theta|; a zero-norm start
+  // replaced by the seeded stream 42+site) before the first, the convergence
+  // check after iteration b-1 when `check`.
+  auto stretch = [&](int a, int b, bool check) {
+    if (a == 0) {
+      lz_reset(st, *lz);
+      lz_norm<T>(st, *lz, theta, n);
+      lz_fill_if_small<T>(st, *lz, theta, n, (uint64_t)(42 + site), 1e-14);
+      lz_norm<T>(st, *lz, theta, n);
+      lz_scale<T>(st, *lz, V, theta, n);
+    }
+    for (int it = a; it < b; ++it) {
+      T* vi = V + (size_t)it * n;
+      if (timed) DMRG_HIP(hipEventRecord(ev[2 * it], st));
+      matvec(vi, work);
+      if (timed) DMRG_HIP(hipEventRecord(ev[2 * it + 1], st));
+      lz_alpha<T>(st, *lz, vi, work, n, it);
+      lz_sub_dots<T>(st, *lz, work, V, n, it);
+      lz_orth_norm<T>(st, *lz, work, V, n, it);
+      lz_next<T>(st, *lz, it + 1 < max_iter ? V + (size_t)(it + 1) * n : nullptr, work, n, it);
+    }
+    if (check) {
+      lz_check(st, *lz, b, beta_tol, eig_tol, eig_rel);
       DMRG_HIP(hipMemcpyAsync(h_status, lz->status, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    }
+  };
+  int niter = max_iter;
+  int a = 0, chunk = 0;
+  while (a < max_iter) {
+    // the reference checks after iterations 6, 9, 12, ... (it >= 4, it % 3 == 0)
+    int b = a;
+    while (b < max_iter && !(b >= 4 && b % 3 == 0)) ++b;
+    const bool check = b < max_iter;
+    if (check) ++b;  // iteration b-1 is the checked one
+    if (use_graph) {
+      const uint64_t key = graph_key * 64 + (uint64_t)chunk;
+      auto itg = graphs.find(key);
+      if (itg != graphs.end() && (itg->second.n != n || itg->second.a != a || itg->second.b != b)) {
+        (void)hipGraphExecDestroy(itg->second.exec);
+        graphs.erase(itg);
+        itg = graphs.end();
+      }
+      if (itg == graphs.end()) {
+        hipGraph_t g = nullptr;
+        DMRG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+          stretch(a, b, check);
+        } catch (...) {
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        DMRG_HIP(hipStreamEndCapture(st, &g));
+        GraphEntry ge;
+        hipError_t e = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DMRG_HIP(e);
+        ge.n = n;
+        ge.a = a;
+        ge.b = b;
+        itg = graphs.emplace(key, ge).first;
+        ++graph_captures;
+      }
+      DMRG_HIP(hipGraphLaunch(itg->second.exec, st));
+      ++graph_launches;
+    } else {
+      stretch(a, b, check);
+    }
+    nmv += b - a;
+    if (check) {
       DMRG_HIP(hipStreamSynchronize(st));
       if (h_status[0]) {
         niter = h_status[1];
         break;
       }
     }
+    a = b;
+    ++chunk;
   }
   lz_final(st, *lz, niter);
   lz_ritz<T>(st, *lz, theta, V, n, niter);
@@ -82,6 +137,13 @@ double LanczosRunner<T>::run(hipStream_t st, T* theta, size_t n, int kmax, int s
     throw Error(ST_NONFINITE, "lanczos: non-finite energy or zero Ritz vector at site " +
                                   std::to_string(site));
   return E;
+}
+
+template <class T>
+void LanczosRunner<T>::clear_graphs() {
+  for (auto& kv : graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  graphs.clear();
 }
 
 // ---------------------------------------------------------------------------
@@ -110,6 +172,7 @@ Engine<T>::Engine(int L_, int d_, int chi_, int D_, double tol_, int variant_, c
                "FP32 matvec mode is available for the real two-site engine");
   DMRG_REQUIRE((long long)chi * d <= 2048, "chi_max * d must be <= 2048");
   DMRG_HIP(hipSetDevice(opt.device));
+  if (opt.autotune) gemm_autotune(true);
   bond.assign(L + 1, 1);
   for (int i = 1; i < L; ++i) {
     int e = std::min(i, L - i);
@@ -174,6 +237,29 @@ void Engine<T>::alloc_all() {
       sc_f.split_bf16 = opt.precision == 2;
       dmalloc((void**)&sc_f.X, sc_f.cap * sizeof(float));
       dmalloc((void**)&sc_f.Y, sc_f.cap * sizeof(float));
+      if (opt.env_precision != 0) {
+        env32 = true;
+        Lfe.assign(L + 1, nullptr);
+        Rfe.assign(L + 1, nullptr);
+        for (int i = 0; i <= L; ++i) {
+          size_t sz = (i == 0 || i == L) ? (size_t)D : env_sz;
+          dmalloc((void**)&Lfe[i], sz * sizeof(float));
+          dmalloc((void**)&Rfe[i], sz * sizeof(float));
+        }
+        Wf.assign(L, nullptr);
+        Wrf.assign(L, nullptr);
+        const size_t wsz = (size_t)D * d * d * D;
+        for (int i = 0; i < L; ++i) {
+          dmalloc((void**)&Wf[i], wsz * sizeof(float));
+          dmalloc((void**)&Wrf[i], wsz * sizeof(float));
+        }
+        dmalloc((void**)&envS, 2 * (size_t)(L + 1) * sizeof(double));
+        dmalloc((void**)&envMax, sizeof(double));
+        dmalloc((void**)&Af, site_sz * sizeof(float));
+        sc_envf.cap = (size_t)D * d * chi * chi;
+        dmalloc((void**)&sc_envf.X, sc_envf.cap * sizeof(float));
+        dmalloc((void**)&sc_envf.Y, sc_envf.cap * sizeof(float));
+      }
     }
   }
   dmalloc((void**)&theta, nmax * sizeof(T));
@@ -187,6 +273,11 @@ void Engine<T>::alloc_all() {
   dmalloc((void**)&sc_env.Y, sc_env.cap * sizeof(T));
   lz.alloc(std::max(kmax, 1), lz_part_doubles(std::max(kmax, 1), nmax) * sizeof(double), sizeof(T));
   svdw.alloc(chi * d);
+  if (opt.svd_rsvd) {
+    DMRG_REQUIRE(opt.rsvd_oversample >= 0 && opt.rsvd_power >= 0 && opt.rsvd_power <= 8,
+                 "rsvd: oversample >= 0 and power in [0, 8]");
+    rsvdw.alloc(chi * d, std::min(chi * d, chi + opt.rsvd_oversample));
+  }
   if (variant == 2 && opt.svd_warm) warm_alloc();
   DMRG_HIP(hipHostMalloc((void**)&h_status, 4 * sizeof(int), hipHostMallocDefault));
   DMRG_HIP(hipHostMalloc((void**)&h_scal, 8 * sizeof(double), hipHostMallocDefault));
@@ -218,6 +309,7 @@ void Engine<T>::alloc_all() {
 
 template <class T>
 void Engine<T>::free_all() {
+  seg_free();
   if (s_main) (void)hipStreamSynchronize(s_main);
   if (s_env) (void)hipStreamSynchronize(s_env);
   auto f = [](void* p) { if (p) (void)hipFree(p); };
@@ -232,6 +324,16 @@ void Engine<T>::free_all() {
   f(Lf); f(Rf); f(thf); f(outf); f(envScale);
   envScale = nullptr;
   f(sc_f.X); f(sc_f.Y);
+  for (auto p : Lfe) f(p);
+  for (auto p : Rfe) f(p);
+  for (auto p : Wf) f(p);
+  for (auto p : Wrf) f(p);
+  Lfe.clear(); Rfe.clear(); Wf.clear(); Wrf.clear();
+  f(envS); f(envMax); f(Af); f(sc_envf.X); f(sc_envf.Y);
+  envS = envMax = nullptr;
+  Af = nullptr;
+  sc_envf.X = sc_envf.Y = nullptr;
+  env32 = false;
   f(theta); f(work); f(V);
   f(sc_main.X); f(sc_main.Y); f(sc_env.X); f(sc_env.Y);
   mps.clear(); Lenv.clear(); Renv.clear(); Wd.clear(); Wrd.clear(); WWd.clear(); WWf.clear();
@@ -239,9 +341,11 @@ void Engine<T>::free_all() {
   sc_f.X = sc_f.Y = nullptr; sc_main.X = sc_main.Y = sc_env.X = sc_env.Y = nullptr;
   lz.release();
   svdw.release();
+  rsvdw.release();
   warm_free();
   for (auto e : runner.ev) (void)hipEventDestroy(e);
   runner.ev.clear();
+  runner.clear_graphs();
   if (h_status) (void)hipHostFree(h_status);
   if (h_scal) (void)hipHostFree(h_scal);
   if (h_aux) (void)hipHostFree(h_aux);
@@ -276,6 +380,13 @@ void Engine<T>::set_mpo(const T* const* W) {
     DMRG_HIP(hipMemcpy(Wd[i], W[i], wsz * sizeof(T), hipMemcpyHostToDevice));
     build_wright<T>(W[i], d, D, wr.data());
     DMRG_HIP(hipMemcpy(Wrd[i], wr.data(), wsz * sizeof(T), hipMemcpyHostToDevice));
+    if (env32) {
+      std::vector<float> wf(wsz);
+      for (size_t k = 0; k < wsz; ++k) wf[k] = (float)real(W[i][k]);
+      DMRG_HIP(hipMemcpy(Wf[i], wf.data(), wsz * sizeof(float), hipMemcpyHostToDevice));
+      for (size_t k = 0; k < wsz; ++k) wf[k] = (float)real(wr[k]);
+      DMRG_HIP(hipMemcpy(Wrf[i], wf.data(), wsz * sizeof(float), hipMemcpyHostToDevice));
+    }
   }
   if (variant == 2) {
     const size_t K = (size_t)D * d * d;
@@ -397,6 +508,48 @@ void Engine<T>::env_right_at(int site) {
   stats.env_flop += flops_env(bond[site + 1], bond[site], d, D, is_cplx<T>::value);
 }
 
+// FP32 environment updates (mixed modes): the same contractions on FP32 MFMA
+// from the FP32 copy of the site tensor, then renormalised to unit max-abs
+// with the factor carried into the new scale.
+template <class T>
+void Engine<T>::env_left_at32(int site) {
+  if constexpr (!is_cplx<T>::value) {
+    const bool timed = env_timing && env_ev_n + 2 <= env_ev.size();
+    if (timed) DMRG_HIP(hipEventRecord(env_ev[env_ev_n], s_env));
+    const int ci = bond[site], co = bond[site + 1];
+    convert_f64_f32(s_env, mps[site], Af, (size_t)ci * d * co);
+    env_left<float>(s_env, Lfe[site], Af, Wf[site], Lfe[site + 1], ci, co, d, D, sc_envf);
+    const size_t n = (size_t)co * D * co;
+    absmax_f32(s_env, Lfe[site + 1], n, envMax);
+    rescale_env_f32(s_env, Lfe[site + 1], n, envMax, envS + site, envS + site + 1);
+    if (timed) {
+      DMRG_HIP(hipEventRecord(env_ev[env_ev_n + 1], s_env));
+      env_ev_n += 2;
+    }
+    stats.env_flop += flops_env(ci, co, d, D, false);
+  }
+}
+
+template <class T>
+void Engine<T>::env_right_at32(int site) {
+  if constexpr (!is_cplx<T>::value) {
+    const bool timed = env_timing && env_ev_n + 2 <= env_ev.size();
+    if (timed) DMRG_HIP(hipEventRecord(env_ev[env_ev_n], s_env));
+    const int ci = bond[site + 1], co = bond[site];
+    convert_f64_f32(s_env, mps[site], Af, (size_t)co * d * ci);
+    env_right<float>(s_env, Rfe[site + 1], Af, Wrf[site], Rfe[site], ci, co, d, D, sc_envf);
+    const size_t n = (size_t)co * D * co;
+    double* const sR = envS + (L + 1);
+    absmax_f32(s_env, Rfe[site], n, envMax);
+    rescale_env_f32(s_env, Rfe[site], n, envMax, sR + site + 1, sR + site);
+    if (timed) {
+      DMRG_HIP(hipEventRecord(env_ev[env_ev_n + 1], s_env));
+      env_ev_n += 2;
+    }
+    stats.env_flop += flops_env(ci, co, d, D, false);
+  }
+}
+
 // Adds the device time of the environment updates recorded since the last
 // call (option profile) to stats.env_ms; the streams must be synchronised.
 template <class T>
@@ -411,9 +564,18 @@ void Engine<T>::flush_env_timing() {
 
 template <class T>
 void Engine<T>::build_envs() {
-  set_boundary_env<T>(s_env, Lenv[0], D, 0);
-  set_boundary_env<T>(s_env, Renv[L], D, D - 1);
-  for (int i = L - 1; i >= 0; --i) env_right_at(i);
+  if (env32) {
+    if constexpr (!is_cplx<T>::value) {
+      set_boundary_env<float>(s_env, Lfe[0], D, 0);
+      set_boundary_env<float>(s_env, Rfe[L], D, D - 1);
+      fill_f64(s_env, envS, 2 * (size_t)(L + 1), 1.0);
+      for (int i = L - 1; i >= 0; --i) env_right_at32(i);
+    }
+  } else {
+    set_boundary_env<T>(s_env, Lenv[0], D, 0);
+    set_boundary_env<T>(s_env, Renv[L], D, D - 1);
+    for (int i = L - 1; i >= 0; --i) env_right_at(i);
+  }
   DMRG_HIP(hipStreamSynchronize(s_env));
   env_pending = false;
 }
@@ -438,8 +600,14 @@ void Engine<T>::matvec2(int site, const T* in, T* out, bool fp64) {
     if constexpr (!is_cplx<T>::value) {
       const size_t n = (size_t)cL * d * d * cR;
       convert_f64_f32(s_main, in, thf, n);
-      heff2_apply<float>(s_main, Lf, Rf, WWf[site], thf, outf, cL, cR, d, D, sc_f);
-      convert_f32_f64_scaled(s_main, outf, out, n, envScale, envScale + 1);
+      if (env32) {
+        heff2_apply<float>(s_main, Lfe[site], Rfe[site + 2], WWf[site], thf, outf, cL, cR, d, D,
+                           sc_f);
+        convert_f32_f64_scaled(s_main, outf, out, n, envS + site, envS + (L + 1) + site + 2);
+      } else {
+        heff2_apply<float>(s_main, Lf, Rf, WWf[site], thf, outf, cL, cR, d, D, sc_f);
+        convert_f32_f64_scaled(s_main, outf, out, n, envScale, envScale + 1);
+      }
     }
   }
 }
@@ -451,21 +619,28 @@ double Engine<T>::lanczos_two(int site, bool rq) {
   if (opt.precision != 0) {
     if constexpr (!is_cplx<T>::value) {
       const size_t nl = (size_t)cL * D * cL, nr = (size_t)cR * D * cR;
-      absmax_f64(s_main, Lenv[site], nl, envScale);
-      absmax_f64(s_main, Renv[site + 2], nr, envScale + 1);
-      convert_f64_f32_scaled(s_main, Lenv[site], Lf, nl, envScale);
-      convert_f64_f32_scaled(s_main, Renv[site + 2], Rf, nr, envScale + 1);
+      if (!env32) {
+        absmax_f64(s_main, Lenv[site], nl, envScale);
+        absmax_f64(s_main, Renv[site + 2], nr, envScale + 1);
+        convert_f64_f32_scaled(s_main, Lenv[site], Lf, nl, envScale);
+        convert_f64_f32_scaled(s_main, Renv[site + 2], Rf, nr, envScale + 1);
+      }
     }
   }
   double t0 = now_s();
   double E = runner.run(s_main, theta, n, opt.krylov_max, site,
-                        [&](const T* in, T* out) { matvec2(site, in, out, false); });
+                        [&](const T* in, T* out) { matvec2(site, in, out, false); },
+                        graph_key(site, cL, cR));
   stats.lanczos_ms += 1e3 * (now_s() - t0);
   stats.heff_ms += runner.last_heff_ms;
   stats.matvecs += runner.last_matvecs;
   stats.lanczos_calls += 1;
   stats.heff_flop += runner.last_matvecs * flops_heff2(cL, cR, d, D, is_cplx<T>::value);
-  if (rq && opt.precision != 0) E = rayleigh2(site);
+  stats.graph_launches = runner.graph_launches;
+  stats.graph_captures = runner.graph_captures;
+  // FP64 environments: the FP64 Rayleigh quotient at the sweep's last bond;
+  // FP32 environments: run() takes the energy from an FP64 contraction
+  if (rq && opt.precision != 0 && !env32) E = rayleigh2(site);
   return E;
 }
 
@@ -490,14 +665,25 @@ void Engine<T>::split_two(int site, char dir) {
   const int full_k = std::min(m, n), k = std::min(full_k, chi);
   double t0 = now_s();
   const int wi = dir == 'R' ? 0 : 1;
-  const T* warm = warm_on ? warm_basis(site, dir) : nullptr;
-  T* keep = warm_on ? wX[wi][site] : nullptr;
-  SvdResult r = svd_split<T>(s_main, svdw, theta, m, n, m, k, opt.svd_cutoff, opt.svd_rel_cutoff,
-                             dir == 'L', mps[site], m, mps[site + 1], nullptr, svd_params(), warm,
-                             keep);
-  if (warm_on) {
-    wXN[wi][site] = dir == 'R' ? m : n;
-    wXB[wi][site] = dir == 'R' ? cL : cR;
+  // the reference's RSVD condition (dmrg2_gpu_impl.h:1186-1188)
+  const bool rsvd = opt.svd_rsvd && full_k > k + opt.rsvd_oversample && m > 2 * k;
+  SvdResult r;
+  if (rsvd) {
+    r = rsvd_split<T>(s_main, svdw, rsvdw, theta, m, n, m, k, opt.svd_cutoff, opt.svd_rel_cutoff,
+                      dir == 'L', mps[site], m, mps[site + 1], svd_params(), opt.rsvd_oversample,
+                      opt.rsvd_power, 0x5EED0000ull + rsvd_seq++);
+    stats.rsvd_calls += 1;
+    if (warm_on) wXN[wi][site] = 0;  // no stored basis for this bond
+  } else {
+    const T* warm = warm_on ? warm_basis(site, dir) : nullptr;
+    T* keep = warm_on ? wX[wi][site] : nullptr;
+    r = svd_split<T>(s_main, svdw, theta, m, n, m, k, opt.svd_cutoff, opt.svd_rel_cutoff,
+                     dir == 'L', mps[site], m, mps[site + 1], nullptr, svd_params(), warm, keep);
+    if (warm_on) {
+      wXN[wi][site] = dir == 'R' ? m : n;
+      wXB[wi][site] = dir == 'R' ? cL : cR;
+    }
+    stats.svd_warm_starts += warm ? 1 : 0;
   }
   bond[site + 1] = r.nk;
   DMRG_HIP(hipEventRecord(ev_canon, s_main));
@@ -507,8 +693,8 @@ void Engine<T>::split_two(int site, char dir) {
   stats.svd_sweeps += r.sweeps;
   stats.svd_bip_sweeps += r.bip_sweeps;
   stats.svd_oa_steps += r.oa_steps;
-  stats.svd_warm_starts += warm ? 1 : 0;
   stats.max_discarded = std::max(stats.max_discarded, r.discarded);
+  stats.trunc_sum_cur += r.discarded;
 }
 
 // ---------------------------------------------------------------------------
@@ -597,8 +783,7 @@ void Engine<T>::warm_after_split(int site, char dir) {
     } else {
       wTr = wTc = 0;
     }
-    DMRG_HIP(hipMemcpyAsync(wA[0][site], Anew, (size_t)cL * d * nk * sizeof(T),
-                            hipMemcpyDeviceToDevice, s_main));
+    copy_dev<T>(s_main, wA[0][site], Anew, (size_t)cL * d * nk);
     wAL[0][site] = cL;
     wAR[0][site] = nk;
   } else {
@@ -613,8 +798,7 @@ void Engine<T>::warm_after_split(int site, char dir) {
     } else {
       wTr = wTc = 0;
     }
-    DMRG_HIP(hipMemcpyAsync(wA[1][site + 1], Bnew, (size_t)nk * d * cR * sizeof(T),
-                            hipMemcpyDeviceToDevice, s_main));
+    copy_dev<T>(s_main, wA[1][site + 1], Bnew, (size_t)nk * d * cR);
     wAL[1][site + 1] = nk;
     wAR[1][site + 1] = cR;
   }
@@ -638,7 +822,8 @@ double Engine<T>::sweep2_lr() {
   for (int site = 0; site < L - 1; ++site) {
     E = optimize_bond(site, 'R', false);
     DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));
-    env_left_at(site);
+    if (env32) env_left_at32(site);
+    else env_left_at(site);
     DMRG_HIP(hipEventRecord(ev_env, s_env));
     env_pending = true;
   }
@@ -653,7 +838,8 @@ double Engine<T>::sweep2_rl() {
   for (int site = L - 2; site >= 0; --site) {
     E = optimize_bond(site, 'L', site == 0);
     DMRG_HIP(hipStreamWaitEvent(s_env, ev_canon, 0));
-    env_right_at(site + 1);
+    if (env32) env_right_at32(site + 1);
+    else env_right_at(site + 1);
     DMRG_HIP(hipEventRecord(ev_env, s_env));
     env_pending = true;
   }
@@ -671,12 +857,14 @@ double Engine<T>::lanczos_one(int site) {
   double t0 = now_s();
   double E = runner.run(s_main, theta, n, opt.krylov_max, site, [&](const T* in, T* out) {
     heff1_apply<T>(s_main, Lenv[site], Renv[site + 1], Wd[site], in, out, cL, cR, d, D, sc_main);
-  });
+  }, graph_key(site, cL, cR));
   stats.lanczos_ms += 1e3 * (now_s() - t0);
   stats.heff_ms += runner.last_heff_ms;
   stats.matvecs += runner.last_matvecs;
   stats.lanczos_calls += 1;
   stats.heff_flop += runner.last_matvecs * flops_heff1(cL, cR, d, D, is_cplx<T>::value);
+  stats.graph_launches = runner.graph_launches;
+  stats.graph_captures = runner.graph_captures;
   return E;
 }
 
@@ -692,6 +880,7 @@ void Engine<T>::split_one(int site, char dir) {
                                opt.svd_rel_cutoff, false, mps[site], m, work, nullptr, svd_params());
     const int nk = r.nk;
     stats.max_discarded = std::max(stats.max_discarded, r.discarded);
+    stats.trunc_sum_cur += r.discarded;
     DMRG_HIP(hipEventRecord(ev_canon, s_main));
     const int cR2 = bond[site + 2];
     gemm<T>(s_main, OP_N, OP_N, nk, d * cR2, cR, T(1), work, nk, mps[site + 1], cR, T(0), mps_spare,
@@ -707,6 +896,7 @@ void Engine<T>::split_one(int site, char dir) {
                                opt.svd_rel_cutoff, true, work, m, mps[site], nullptr, svd_params());
     const int nk = r.nk;
     stats.max_discarded = std::max(stats.max_discarded, r.discarded);
+    stats.trunc_sum_cur += r.discarded;
     DMRG_HIP(hipEventRecord(ev_canon, s_main));
     const int cL0 = bond[site - 1];
     gemm<T>(s_main, OP_N, OP_N, cL0 * d, nk, cL, T(1), mps[site - 1], cL0 * d, work, cL, T(0),
@@ -723,12 +913,12 @@ template <class T>
 double Engine<T>::optimize_site(int site, char dir, bool split) {
   wait_env();
   const size_t n = (size_t)bond[site] * d * bond[site + 1];
-  DMRG_HIP(hipMemcpyAsync(theta, mps[site], n * sizeof(T), hipMemcpyDeviceToDevice, s_main));
+  copy_dev<T>(s_main, theta, mps[site], n);
   double E = lanczos_one(site);
   if (split) {
     split_one(site, dir);
   } else {
-    DMRG_HIP(hipMemcpyAsync(mps[site], theta, n * sizeof(T), hipMemcpyDeviceToDevice, s_main));
+    copy_dev<T>(s_main, mps[site], theta, n);
   }
   return E;
 }
@@ -784,6 +974,7 @@ double Engine<T>::run(int n_sweeps, bool quiet) {
   } timing_off{this};
   for (int sweep = 0; sweep < n_sweeps; ++sweep) {
     double ts = now_s();
+    stats.trunc_sum_cur = 0.0;
     if (variant == 2) {
       sweep2_lr();
       energy = sweep2_rl();
@@ -791,9 +982,18 @@ double Engine<T>::run(int n_sweeps, bool quiet) {
       sweep1_lr();
       energy = sweep1_rl();
     }
+    if (env32) {
+      // FP32 environments: the sweep energy is the FP64 <psi|H|psi>/<psi|psi>
+      // of the state (the last bond of the R->L half-sweep keeps every
+      // singular value, so this is the Rayleigh quotient of its Ritz vector)
+      double e = 0.0, n2 = 0.0;
+      expectation(&e, &n2);
+      energy = e;
+    }
     DMRG_HIP(hipStreamSynchronize(s_main));
     DMRG_HIP(hipStreamSynchronize(s_env));
     stats.sweep_s_last = now_s() - ts;
+    stats.trunc_sum_last = stats.trunc_sum_cur;
     flush_env_timing();
     stats.sweeps_s_total += stats.sweep_s_last;
     stats.sweeps_done += 1;
@@ -850,8 +1050,19 @@ int Engine<T>::get_env(bool left, int idx, T* out) {
   if (out) {
     DMRG_HIP(hipStreamSynchronize(s_main));
     DMRG_HIP(hipStreamSynchronize(s_env));
-    DMRG_HIP(hipMemcpy(out, left ? Lenv[idx] : Renv[idx], (size_t)c * D * c * sizeof(T),
-                       hipMemcpyDeviceToHost));
+    const size_t n = (size_t)c * D * c;
+    if (env32) {
+      // FP32 environment times its scale
+      std::vector<float> hf(n);
+      double sc = 1.0;
+      DMRG_HIP(hipMemcpy(hf.data(), left ? Lfe[idx] : Rfe[idx], n * sizeof(float),
+                         hipMemcpyDeviceToHost));
+      DMRG_HIP(hipMemcpy(&sc, envS + (left ? 0 : L + 1) + idx, sizeof(double),
+                         hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) out[i] = T((double)hf[i] * sc);
+    } else {
+      DMRG_HIP(hipMemcpy(out, left ? Lenv[idx] : Renv[idx], n * sizeof(T), hipMemcpyDeviceToHost));
+    }
   }
   return c;
 }

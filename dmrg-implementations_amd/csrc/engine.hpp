@@ -4,6 +4,7 @@
 #pragma once
 #include <chrono>
 #include <functional>
+#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -34,6 +35,26 @@ struct Options {
   int svd_subspace = 1;
   // Jacobi stop of the split in the FP32 / BF16X3 matvec modes.
   double svd_tol_mixed = 1e-13;
+  // FP32 / BF16X3 modes: 1 keeps the environments in FP32 (scaled) and runs
+  // their updates on FP32 MFMA, the sweep energy then coming from an FP64
+  // contraction of the state; 0 (default) keeps FP64 environments, converted
+  // per bond.  FP32 environments accumulate FP32 rounding along the chain
+  // (~1e-6 of H_eff against ~1e-7 from the matvec alone), which measured
+  // |dE|/|E| = 2.4e-10 against FP64 at Heisenberg L=128, chi=512 after 10
+  // sweeps, against 2e-12 with FP64 environments.
+  int env_precision = 0;
+  // Randomised split (the reference's RSVD option): used for a two-site
+  // split when it keeps fewer than min(m, n) - oversample values and
+  // m > 2 k, as in the reference; power = subspace iterations.
+  int svd_rsvd = 0;
+  int rsvd_oversample = 10;
+  int rsvd_power = 0;
+  // Replay each bond's Lanczos iterations between two convergence checks as
+  // a captured HIP graph (cached per bond and bond dimensions): one launch
+  // instead of ~8 per iteration.  Off with profile (events are not captured).
+  int graph = 0;
+  // Turn on the process-wide GEMM tile autotuner (gemm_autotune) at creation.
+  int autotune = 0;
 };
 
 struct Stats {
@@ -45,6 +66,11 @@ struct Stats {
   double env_build_s = 0, sweeps_s_total = 0;
   long long svd_warm_starts = 0;
   long long svd_bip_sweeps = 0, svd_oa_steps = 0;
+  long long rsvd_calls = 0;
+  long long graph_launches = 0, graph_captures = 0;
+  long long tuned_shapes = 0;  // GEMM shapes the autotuner has timed (process-wide)
+  double trunc_sum_last = 0;  // discarded weight summed over the last sweep's splits
+  double trunc_sum_cur = 0;   // running sum of the sweep in progress
 };
 
 // Algorithmic flop counts (real arithmetic; complex = 4x).
@@ -76,8 +102,50 @@ struct LanczosRunner {
   double beta_tol = 1e-12, eig_tol = 1e-12, eig_rel = 0.0;
   int last_niter = 0, last_matvecs = 0;
   double last_heff_ms = 0;
+  // graph_key != 0 (and not timed): each stretch of iterations between two
+  // convergence checks runs as a HIP graph captured on first use and cached
+  // under (graph_key, stretch); the key must change whenever anything the
+  // stretch's launches read as an argument changes (site, bond dimensions).
   double run(hipStream_t st, T* theta, size_t n, int kmax, int site,
-             const std::function<void(const T*, T*)>& matvec);
+             const std::function<void(const T*, T*)>& matvec, uint64_t graph_key = 0);
+  struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    size_t n = 0;
+    int a = 0, b = 0;
+  };
+  std::unordered_map<uint64_t, GraphEntry> graphs;
+  long long graph_launches = 0, graph_captures = 0;
+  void clear_graphs();
+};
+
+// Per-segment work space of the segment-parallel sweep (pdmrg.hip): its own
+// stream, two-site vectors, Krylov basis, contraction scratch, split and
+// Lanczos state, so segments run concurrently on one device.
+template <class T>
+struct SegWork {
+  hipStream_t st = nullptr;
+  T *theta = nullptr, *work = nullptr, *V = nullptr, *tmp = nullptr;
+  double* sigma = nullptr;  // singular values of a boundary split
+  ContractScratch<T> sc;
+  SvdWork<T> svd;
+  LzDev lz;
+  LanczosRunner<T> runner;
+  int* h_status = nullptr;
+  double* h_scal = nullptr;
+  Stats stats;
+  double energy = 0.0;
+  void alloc(int chi, int d, int D, int kmax, const Options& opt);
+  void release();
+};
+
+// Options of the segment-parallel run (the reference's PDMRGGPU::run
+// arguments, gpu-rocm/pdmrg-gpu/src/pdmrg_gpu.h:40-41).
+struct PdmrgParams {
+  int segments = 2;
+  int outer = 10;   // outer iterations (stop when the energy changes by < tol)
+  int local = 2;    // staggered segment sweeps + boundary merges per outer iteration
+  int warmup = 1;   // serial full sweeps before the parallel phase
+  int polish = 1;   // serial full sweeps after it
 };
 
 template <class T>
@@ -94,6 +162,9 @@ class Engine {
   void init_neel();
   void set_mps(int site, const T* A, int cL, int cR);
   double run(int n_sweeps, bool quiet);
+  // Segment-parallel sweep (PDMRG, the reference's pdmrg-gpu,
+  // gpu-rocm/pdmrg-gpu/src/pdmrg_gpu_impl.h:2407-2753), two-site FP64.
+  double run_pdmrg(const PdmrgParams& p, bool quiet);
   void get_mps(int site, T* out) const;
   void expectation(double* e, double* nrm2);
   int get_env(bool left, int idx, T* out);
@@ -120,6 +191,17 @@ class Engine {
   void env_left_at(int site);
   void env_right_at(int site);  // builds R[site] from R[site+1] and A[site]
   void flush_env_timing();
+  // FP32 environments (mixed modes with env_precision = 1): Lfe/Rfe hold
+  // the unit-max float arrays, envS the scales (envS[i] of Lfe[i],
+  // envS[L + 1 + i] of Rfe[i]); Af stages one site tensor in FP32.
+  bool env32 = false;
+  std::vector<float*> Lfe, Rfe, Wf, Wrf;
+  double* envS = nullptr;
+  double* envMax = nullptr;
+  float* Af = nullptr;
+  ContractScratch<float> sc_envf;
+  void env_left_at32(int site);
+  void env_right_at32(int site);
   std::vector<hipEvent_t> env_ev;  // option profile: event pairs around env updates
   size_t env_ev_n = 0;
   bool env_timing = false;
@@ -141,6 +223,20 @@ class Engine {
   void warm_after_split(int site, char dir);
   void split_one(int site, char dir);
   double rayleigh2(int site);
+  // Graph-cache key of a bond's Lanczos launches (0: no graphs): every
+  // argument they take is fixed by the site and its bond dimensions.
+  uint64_t graph_key(int site, int cL, int cR) const {
+    if (!opt.graph || opt.profile) return 0;
+    return 1 + (uint64_t)site + (uint64_t)(L + 1) * ((uint64_t)cL + (uint64_t)(chi + 1) * cR);
+  }
+
+  // segment-parallel run (pdmrg.hip)
+  std::vector<SegWork<T>*> segw;
+  void seg_free();
+  double seg_bond(SegWork<T>& w, int site, char dir, const double* vb);
+  void seg_sweep(SegWork<T>& w, int first, int last, char dir);
+  std::vector<double*> pd_V;  // per boundary: V = 1/clip(S) on the device
+  std::vector<int> pd_Vn;     // its length (the boundary bond dimension)
 
   hipStream_t s_main = nullptr, s_env = nullptr;
   hipEvent_t ev_canon = nullptr, ev_env = nullptr;
@@ -158,6 +254,8 @@ class Engine {
   ContractScratch<float> sc_f;
   LzDev lz;
   SvdWork<T> svdw;
+  RsvdWork<T> rsvdw;
+  uint64_t rsvd_seq = 0;  // RSVD calls so far: the sketch of call j uses stream 0x5EED0000 + j
   SvdParams svd_params() const {
     SvdParams p;
     p.max_sweeps = opt.svd_max_sweeps;

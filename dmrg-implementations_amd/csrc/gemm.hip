@@ -13,7 +13,10 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 
 namespace dmrg {
 namespace {
@@ -60,28 +63,36 @@ struct AddrTensor {
   __device__ long long operator()(int r, int c) const { return dim(v.row, r) + dim(v.col, c); }
 };
 
-template <class T, bool CA, bool CB, bool KFA, bool KFB, int BM, int BN, int BK, int PF, class AA,
-          class AB, class AC>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T* __restrict__ A,
-                                              AA aa, long long sA, const T* __restrict__ B, AB ab,
-                                              long long sB, T beta, T* __restrict__ C, AC ac,
-                                              long long sC, int herm) {
+// KS = 2: eight waves, two per output sub-tile, each multiplying one half of
+// every k-tile; the halves are summed through LDS at the end (twice the waves
+// per SIMD to hide latency when a grid has about one workgroup per CU).
+template <class T, bool CA, bool CB, bool KFA, bool KFB, int BM, int BN, int BK, int PF, int KS,
+          class AA, class AB, class AC>
+__global__ __launch_bounds__(256 * KS) void k_gemm(int M, int N, int K, T alpha,
+                                                   const T* __restrict__ A, AA aa, long long sA,
+                                                   const T* __restrict__ B, AB ab, long long sB,
+                                                   T beta, T* __restrict__ C, AC ac, long long sC,
+                                                   int herm) {
   using E = typename GT<T>::E;
   using V4 = typename GT<T>::V4;
   constexpr int P = GT<T>::P;
+  constexpr int NT = 256 * KS;
   constexpr int PADM = BM + 1, PADN = BN + 1;
   constexpr int FM = BM / 32, FN = BN / 32;  // 16x16 blocks per wave
-  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
+  static_assert(KS == 1 || KS == 2, "k split");
+  static_assert(NA >= 1 && NB >= 1 && BK % (4 * KS) == 0, "tile too small for the k split");
 
   __shared__ E As[2][P][BK][PADM];
   __shared__ E Bs[2][P][BK][PADN];
+  __shared__ E Red[KS > 1 ? 4 * P * FM * FN * 4 * 64 : 1];
 
   // Hermitian product (M == N, BM == BN, beta == 0): only the tiles on and
   // above the diagonal are computed; each writes its conjugate mirror too.
   if (herm && blockIdx.x > blockIdx.y) return;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave & 1, wn = (wave >> 1) & 1, wk = wave >> 2;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   A += blockIdx.z * sA;
   B += blockIdx.z * sB;
@@ -91,12 +102,12 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
   T ra[PF][NA], rb[PF][NB];
 
   auto idxA = [&](int r, int& m, int& k) {
-    if (!KFA) { m = t % BM; k = t / BM + r * (256 / BM); }
-    else      { k = t % BK; m = t / BK + r * (256 / BK); }
+    if (!KFA) { m = t % BM; k = t / BM + r * (NT / BM); }
+    else      { k = t % BK; m = t / BK + r * (NT / BK); }
   };
   auto idxB = [&](int r, int& n, int& k) {
-    if (!KFB) { n = t % BN; k = t / BN + r * (256 / BN); }
-    else      { k = t % BK; n = t / BK + r * (256 / BK); }
+    if (!KFB) { n = t % BN; k = t / BN + r * (NT / BN); }
+    else      { k = t % BK; n = t / BK + r * (NT / BK); }
   };
   // register stage `s` <- operand tiles at depth k0; a tile inside all three
   // bounds loads without per-element checks (one branch for the whole stage)
@@ -172,7 +183,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
   const int li = lane & 15, lk = lane >> 4;
   auto compute = [&](int buf) {
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
+    for (int kk = wk * (BK / KS); kk < (wk + 1) * (BK / KS); kk += 4) {
       E af[P][FM], bf[P][FN];
 #pragma unroll
       for (int p = 0; p < P; ++p) {
@@ -227,6 +238,31 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
     }
   }
 
+  if constexpr (KS > 1) {
+    // the second k-half's partial sums join the first's (the loop ended at
+    // a barrier, so no wave reads As/Bs any more)
+    E* red = Red + (size_t)(wave & 3) * (P * FM * FN * 4 * 64);
+    if (wk == 1) {
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[(((p * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[p][i][j][r];
+    }
+    __syncthreads();
+    if (wk == 1) return;
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[p][i][j][r] += red[(((p * FM + i) * FN + j) * 4 + r) * 64 + lane];
+  }
   // Epilogue: lane -> row li of C; register r -> column lk + 4r (f64) or
   // 4*lk + r (f32) of each 16x16 block.
   const bool use_beta = !(real(beta) == 0.0 && imag(beta) == 0.0);
@@ -256,9 +292,14 @@ void launch(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* 
             long long sA, const T* B, AB ab, long long sB, T beta, T* C, AC ac, long long sC,
             int batch, int herm = 0) {
 #define DMRG_GEMM_LAUNCH(BM_, BN_, BK_, PF_)                                                \
-  hipLaunchKernelGGL((k_gemm<T, CA, CB, KFA, KFB, BM_, BN_, BK_, PF_, AA, AB, AC>),         \
-                     dim3(cdiv(M, BM_), cdiv(N, BN_), batch), dim3(256), 0, st, M, N, K,    \
-                     alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC, herm)
+  if (tl.ks == 2)                                                                          \
+    hipLaunchKernelGGL((k_gemm<T, CA, CB, KFA, KFB, BM_, BN_, BK_, PF_, 2, AA, AB, AC>),    \
+                       dim3(cdiv(M, BM_), cdiv(N, BN_), batch), dim3(512), 0, st, M, N, K,  \
+                       alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC, herm);                 \
+  else                                                                                     \
+    hipLaunchKernelGGL((k_gemm<T, CA, CB, KFA, KFB, BM_, BN_, BK_, PF_, 1, AA, AB, AC>),    \
+                       dim3(cdiv(M, BM_), cdiv(N, BN_), batch), dim3(256), 0, st, M, N, K,  \
+                       alpha, A, aa, sA, B, ab, sB, beta, C, ac, sC, herm)
   // 32x32 tiles serve grids that cannot fill the chip with 64x64 ones; a deep
   // k-step and two register stages then hide the load latency that the low
   // tile count per CU leaves exposed.
@@ -287,12 +328,9 @@ void plain4(hipStream_t st, GemmTile tl, int M, int N, int K, T alpha, const T* 
 }
 
 template <class T>
-void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T alpha, const T* A,
-           int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc,
-           long long sC, int batch, int herm = 0) {
-  DMRG_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "gemm: negative size");
-  DMRG_REQUIRE(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "gemm: bad op");
-  if (M == 0 || N == 0 || batch == 0) return;
+void dispatch(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T alpha,
+              const T* A, int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C,
+              int ldc, long long sC, int batch, int herm) {
   const bool ta = op_trans(opA), tb = op_trans(opB);
   bool ca = op_conj(opA), cb = op_conj(opB);
   if (!is_cplx<T>::value) ca = cb = false;
@@ -326,16 +364,127 @@ void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T
 #undef DMRG_P4
 }
 
+// ---------------------------------------------------------------------------
+// Tile autotuner.  When enabled (gemm_autotune), the first non-batched call
+// of each (scalar type, ops, M, N, K, Hermitian) shape times every tile
+// (32x32 and 64x64, with four or eight waves) on
+// the caller's stream (two launches each, into a scratch output, after one
+// warm-up) and keeps the fastest for the rest of the process; later calls
+// of that shape use it.  A call made while its stream is being captured
+// into a graph uses the caller's tile and records nothing.
+// ---------------------------------------------------------------------------
+struct TuneKey {
+  int dt, opA, opB, M, N, K, herm;
+  bool operator==(const TuneKey& o) const {
+    return dt == o.dt && opA == o.opA && opB == o.opB && M == o.M && N == o.N && K == o.K &&
+           herm == o.herm;
+  }
+};
+struct TuneKeyHash {
+  size_t operator()(const TuneKey& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for (int v : {k.dt, k.opA, k.opB, k.M, k.N, k.K, k.herm}) {
+      h ^= (uint64_t)(uint32_t)v;
+      h *= 1099511628211ull;
+    }
+    return (size_t)h;
+  }
+};
+std::mutex g_tune_mu;
+std::unordered_map<TuneKey, GemmTile, TuneKeyHash> g_tune;
+std::atomic<int> g_tune_on{0};
+constexpr int kTuneDevices = 64;
+void* g_tune_buf[kTuneDevices] = {};
+size_t g_tune_cap[kTuneDevices] = {};
+
+template <class T> constexpr int dt_code() { return is_cplx<T>::value ? 1 : (std::is_same<T, float>::value ? 2 : 0); }
+
+template <class T>
+GemmTile tuned_tile(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K,
+                    const T* A, int lda, const T* B, int ldb, int herm) {
+  const TuneKey key{dt_code<T>(), opA, opB, M, N, K, herm};
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  DMRG_HIP(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return tl;
+  int dev = 0;
+  DMRG_HIP(hipGetDevice(&dev));
+  DMRG_REQUIRE(dev >= 0 && dev < kTuneDevices, "gemm autotune: device ordinal out of range");
+  const size_t bytes = (size_t)M * N * sizeof(T);
+  if (g_tune_cap[dev] < bytes) {
+    if (g_tune_buf[dev]) DMRG_HIP(hipFree(g_tune_buf[dev]));
+    g_tune_buf[dev] = nullptr;
+    g_tune_cap[dev] = 0;
+    DMRG_HIP(hipMalloc(&g_tune_buf[dev], bytes));
+    g_tune_cap[dev] = bytes;
+  }
+  T* Cs = static_cast<T*>(g_tune_buf[dev]);
+  hipEvent_t e0, e1;
+  DMRG_HIP(hipEventCreate(&e0));
+  DMRG_HIP(hipEventCreate(&e1));
+  const GemmTile cand[4] = {{32, 32, 1}, {32, 32, 2}, {64, 64, 1}, {64, 64, 2}};
+  GemmTile best = tl;
+  float best_ms = 1e30f;
+  try {
+    for (const GemmTile& c : cand) {
+      dispatch<T>(st, opA, opB, c, M, N, K, T(1), A, lda, 0, B, ldb, 0, T(0), Cs, M, 0, 1, herm);
+      DMRG_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < 2; ++r)
+        dispatch<T>(st, opA, opB, c, M, N, K, T(1), A, lda, 0, B, ldb, 0, T(0), Cs, M, 0, 1, herm);
+      DMRG_HIP(hipEventRecord(e1, st));
+      DMRG_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      DMRG_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g_tune.emplace(key, best);
+  return best;
+}
+
+template <class T>
+void plain(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K, T alpha, const T* A,
+           int lda, long long sA, const T* B, int ldb, long long sB, T beta, T* C, int ldc,
+           long long sC, int batch, int herm = 0) {
+  DMRG_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "gemm: negative size");
+  DMRG_REQUIRE(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "gemm: bad op");
+  if (M == 0 || N == 0 || batch == 0) return;
+  if (batch == 1 && g_tune_on.load(std::memory_order_relaxed))
+    tl = tuned_tile<T>(st, opA, opB, tl, M, N, K, A, lda, B, ldb, herm);
+  dispatch<T>(st, opA, opB, tl, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+              herm);
+}
+
 }  // namespace
+
+void gemm_autotune(bool on) { g_tune_on.store(on ? 1 : 0); }
+int gemm_autotune_entries() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return (int)g_tune.size();
+}
 
 GemmTile gemm_pick_tile(int M, int N, int K, bool complex_) {
   (void)K;
   const long long t64 = (long long)cdiv(M, 64) * cdiv(N, 64);
+  const long long t32 = (long long)cdiv(M, 32) * cdiv(N, 32);
   // 64x64 once there are about four tiles per CU (256 CUs); complex too
-  // (measured at 1280^3: 32x32 with two stages 53.8 TF/s, 64x64 48.3)
+  // (measured at 1280^3: 32x32 with two stages 53.8 TF/s, 64x64 48.3).
+  // At most one 32x32 tile per CU: eight waves splitting every k-tile
+  // (512^3: 24.5 TF/s against 21.3 with four waves).
   (void)complex_;
-  if (t64 < 1024) return {32, 32};
-  return {64, 64};
+  if (t64 >= 1024) return {64, 64, 1};
+  return {32, 32, t32 <= 256 ? 2 : 1};
 }
 
 namespace {

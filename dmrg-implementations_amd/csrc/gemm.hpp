@@ -21,6 +21,7 @@ namespace dmrg {
 
 struct GemmTile {
   int bm, bn;  // workgroup tile: 32x32 or 64x64
+  int ks = 1;  // 2: eight waves, each pair splitting every k-tile in halves
 };
 
 // Index map of one operand dimension: i -> (i % n1) * s1 + (i / n1) * s2.
@@ -88,5 +89,14 @@ bool gemm_nn_splitk(hipStream_t st, int M, int N, int K, const T* A, int lda, co
 
 // Pick a tile so that the grid covers the chip.
 GemmTile gemm_pick_tile(int M, int N, int K, bool complex_);
+
+// Process-wide tile autotuner of gemm / gemm_herm: when on, the first call of
+// each shape (scalar type, ops, M, N, K) times the 32x32 and 64x64 tiles,
+// each with four waves or eight (k-split), on the caller's stream and keeps
+// the fastest for the rest of the process
+// (the tile argument is then only the fallback of calls made during graph
+// capture).  gemm_autotune_entries() counts the shapes tuned so far.
+void gemm_autotune(bool on);
+int gemm_autotune_entries();
 
 }  // namespace dmrg

@@ -1,6 +1,8 @@
 // Density-matrix truncated SVD and block-Jacobi Hermitian eigen-solver
 // (see svd.hpp).
 #include "svd.hpp"
+#include "contract.hpp"
+#include "lanczos.hpp"
 #include "gemm.hpp"
 
 #include <algorithm>
@@ -953,6 +955,70 @@ __global__ void k_add_diag(T* __restrict__ X, int n, double b) {
 }
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+// Modified Gram-Schmidt, twice, on the b <= MGS_B columns of one block of
+// Y (len rows, ld ldy), by one workgroup: for each column j, the norm and
+// the inner products with the later columns come from one pass over the
+// rows, the later columns are projected, and column j is normalised.  A
+// column whose norm falls below `dep` times its norm before the
+// orthogonalisation (nrm0, device) is numerically dependent: it is set to
+// zero (the caller's sketch then simply carries fewer directions).
+constexpr int MGS_B = 32;
+template <class T>
+__global__ __launch_bounds__(256) void k_mgs_block(T* __restrict__ Y, int len, int ldy, int b,
+                                                   const double* __restrict__ nrm0, double dep) {
+  __shared__ T red[MGS_B][8];  // per-wave partial sums (4 waves) of <y_j, y_c>
+  __shared__ double nred[8];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int j = 0; j < b; ++j) {
+      T* yj = Y + (size_t)j * ldy;
+      // |y_j|^2 and <y_j, y_c> for c > j (unnormalised y_j)
+      double nn = 0.0;
+      T dot[MGS_B];
+#pragma unroll
+      for (int c = 0; c < MGS_B; ++c) dot[c] = T(0);
+      for (int i = t; i < len; i += 256) {
+        const T v = yj[i];
+        nn += abs2(v);
+        const T cv = conj(v);
+#pragma unroll
+        for (int c = 0; c < MGS_B; ++c)
+          if (c > j && c < b) dot[c] += cv * Y[i + (size_t)c * ldy];
+      }
+      for (int o = 32; o > 0; o >>= 1) nn += __shfl_xor(nn, o);
+#pragma unroll
+      for (int c = 0; c < MGS_B; ++c) {
+        if (c > j && c < b) {
+          double re = real(dot[c]), im = imag(dot[c]);
+          for (int o = 32; o > 0; o >>= 1) {
+            re += __shfl_xor(re, o);
+            im += __shfl_xor(im, o);
+          }
+          if (lane == 0) red[c][wv] = make_scalar<T>(re, im);
+        }
+      }
+      if (lane == 0) nred[wv] = nn;
+      __syncthreads();
+      const double n2 = nred[0] + nred[1] + nred[2] + nred[3];
+      const double nrm = sqrt(n2);
+      // pass 0 against the norm before any orthogonalisation, pass 1
+      // against the unit norm that pass 0 left
+      const bool keep = n2 > 0.0 && nrm > dep * (pass == 0 ? nrm0[j] : 1.0);
+      const double inv = keep ? 1.0 / nrm : 0.0;
+      // y_c -= (<y_j, y_c> / |y_j|^2) y_j, then y_j /= |y_j| (or 0)
+      for (int c = j + 1; c < b; ++c) {
+        const T d = red[c][0] + red[c][1] + red[c][2] + red[c][3];
+        const T coef = keep ? d * (inv * inv) : T(0);
+        T* yc = Y + (size_t)c * ldy;
+        for (int i = t; i < len; i += 256) yc[i] -= coef * yj[i];
+      }
+      __syncthreads();  // every column update has read y_j
+      for (int i = t; i < len; i += 256) yj[i] = yj[i] * inv;
+      __syncthreads();
+    }
+  }
+}
+
 inline unsigned grid_for(size_t tot) { return (unsigned)std::min<size_t>(cdiv(tot, 256), 8192); }
 
 // Block-Jacobi eigen-decomposition of the Hermitian matrix already stored in
@@ -1350,8 +1416,7 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
       gemm<T>(st, OP_N, OP_H, k, k, len, T(1), Yv, ldy, Qc, ldq, T(0), C, k,
               gemm_pick_tile(k, k, len, is_cplx<T>::value));
       hipLaunchKernelGGL(k_mask_tri<T>, dim3(grid_for((size_t)k * k)), dim3(256), 0, st, C, k, 1);
-      DMRG_HIP(hipMemcpy2DAsync(R, (size_t)k * sizeof(T), Yv, (size_t)ldy * sizeof(T),
-                                (size_t)k * sizeof(T), len, hipMemcpyDeviceToDevice, st));
+      copy_dev<T>(st, R, (size_t)k, Yv, (size_t)ldy, (size_t)k, (size_t)len);
       gemm<T>(st, OP_N, OP_N, k, len, k, T(-1), C, k, Qc, ldq, T(1), R, k,
               gemm_pick_tile(k, len, k, is_cplx<T>::value));
       hipLaunchKernelGGL(k_vecnorm<T>, dim3(k), dim3(256), 0, st, (const T*)R, k, len, 1, w.sigma);
@@ -1359,8 +1424,7 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
       gemm<T>(st, OP_H, OP_N, k, k, len, T(1), Qc, ldq, Yv, ldy, T(0), C, k,
               gemm_pick_tile(k, k, len, is_cplx<T>::value));
       hipLaunchKernelGGL(k_mask_tri<T>, dim3(grid_for((size_t)k * k)), dim3(256), 0, st, C, k, 0);
-      DMRG_HIP(hipMemcpy2DAsync(R, (size_t)m * sizeof(T), Yv, (size_t)ldy * sizeof(T),
-                                (size_t)m * sizeof(T), k, hipMemcpyDeviceToDevice, st));
+      copy_dev<T>(st, R, (size_t)m, Yv, (size_t)ldy, (size_t)m, (size_t)k);
       gemm<T>(st, OP_N, OP_N, len, k, k, T(-1), Qc, ldq, C, k, T(1), R, m,
               gemm_pick_tile(len, k, k, is_cplx<T>::value));
       hipLaunchKernelGGL(k_vecnorm<T>, dim3(k), dim3(256), 0, st, (const T*)R, m, len, 0, w.sigma);
@@ -1395,7 +1459,7 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
     hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)n * nk)), dim3(256), 0, st,
                        (const T*)w.V[cur], Np, (const int*)w.perm, n, nk, 1, right, nk);
   if (sigma_out)
-    DMRG_HIP(hipMemcpyAsync(sigma_out, w.sigma, nk * sizeof(double), hipMemcpyDeviceToDevice, st));
+    copy_dev<double>(st, sigma_out, (const double*)w.sigma, (size_t)nk);
   DMRG_LAUNCH_CHECK();
   return res;
 }
@@ -1406,8 +1470,7 @@ int heig(hipStream_t st, SvdWork<T>& w, const T* H, int N, int ldh, double* eval
   DMRG_REQUIRE(N > 0 && ldh >= N && lde >= N, "heig: bad shape");
   const int Np = round_up(N, JP);
   DMRG_REQUIRE(Np <= w.cap, "heig: matrix exceeds workspace");
-  DMRG_HIP(hipMemcpy2DAsync(w.A[0], (size_t)Np * sizeof(T), H, (size_t)ldh * sizeof(T),
-                            (size_t)N * sizeof(T), N, hipMemcpyDeviceToDevice, st));
+  copy_dev<T>(st, w.A[0], (size_t)Np, H, (size_t)ldh, (size_t)N, (size_t)N);
   JacobiCount jc;
   double off = 0.0;
   const int cur = jacobi<T>(st, w, N, Np, prm, jc, off);
@@ -1420,6 +1483,112 @@ int heig(hipStream_t st, SvdWork<T>& w, const T* H, int N, int ldh, double* eval
   return sweeps;
 }
 
+// Orthonormal basis of the columns of Y (len x r, ld ldy), in place, by
+// block classical Gram-Schmidt with reorthogonalisation (each block of
+// MGS_B columns projected twice against the finished ones by GEMMs, then
+// MGS twice inside the block).  Numerically dependent columns become zero.
+// C (>= r * MGS_B scalars) and nrm (>= r doubles) are device work space.
+template <class T>
+void orth_columns(hipStream_t st, T* Y, int len, int r, int ldy, T* C, double* nrm) {
+  const bool cx = is_cplx<T>::value;
+  hipLaunchKernelGGL(k_vecnorm<T>, dim3(r), dim3(256), 0, st, (const T*)Y, ldy, len, 0, nrm);
+  DMRG_LAUNCH_CHECK();
+  for (int j0 = 0; j0 < r; j0 += MGS_B) {
+    const int b = std::min(MGS_B, r - j0);
+    T* Yj = Y + (size_t)j0 * ldy;
+    for (int rep = 0; rep < 2 && j0 > 0; ++rep) {
+      // C (j0 x b) = Q^H Y_j;  Y_j -= Q C
+      gemm<T>(st, OP_H, OP_N, j0, b, len, T(1), Y, ldy, Yj, ldy, T(0), C, j0,
+              gemm_pick_tile(j0, b, len, cx));
+      gemm<T>(st, OP_N, OP_N, len, b, j0, T(-1), Y, ldy, C, j0, T(1), Yj, ldy,
+              gemm_pick_tile(len, b, j0, cx));
+    }
+    hipLaunchKernelGGL(k_mgs_block<T>, dim3(1), dim3(256), 0, st, Yj, len, ldy, b,
+                       (const double*)(nrm + j0), 1e-13);
+    DMRG_LAUNCH_CHECK();
+  }
+}
+
+template <class T>
+void RsvdWork<T>::alloc(int cap_len, int cap_r) {
+  release();
+  const size_t big = (size_t)cap_len * cap_r;
+  DMRG_HIP(hipMalloc(&Om, big * sizeof(T)));
+  DMRG_HIP(hipMalloc(&Y, big * sizeof(T)));
+  DMRG_HIP(hipMalloc(&Z, big * sizeof(T)));
+  DMRG_HIP(hipMalloc(&B, big * sizeof(T)));
+  DMRG_HIP(hipMalloc(&Wb, big * sizeof(T)));
+  DMRG_HIP(hipMalloc(&C, (size_t)cap_r * MGS_B * sizeof(T)));
+  DMRG_HIP(hipMalloc(&nrm, (size_t)cap_r * sizeof(double)));
+  len = cap_len;
+  r = cap_r;
+}
+
+template <class T>
+void RsvdWork<T>::release() {
+  for (T** p : {&Om, &Y, &Z, &B, &Wb, &C}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (nrm) (void)hipFree(nrm);
+  nrm = nullptr;
+  len = r = 0;
+}
+
+template <class T>
+SvdResult rsvd_split(hipStream_t st, SvdWork<T>& w, RsvdWork<T>& rw, const T* A, int m, int n,
+                     int lda, int kmax, double cutoff, double rel_cutoff, bool scale_left, T* left,
+                     int ldl, T* right, const SvdParams& prm, int oversample, int power,
+                     uint64_t seed) {
+  DMRG_REQUIRE(m > 0 && n > 0 && lda >= m && kmax >= 1 && oversample >= 0 && power >= 0,
+               "rsvd: bad shape");
+  const bool cx = is_cplx<T>::value;
+  const int k = std::min(std::min(m, n), kmax);
+  const int r = std::min(k + oversample, std::min(m, n));
+  const int len_s = scale_left ? n : m;  // side of the orthonormal factor
+  const int len_o = scale_left ? m : n;  // the other side
+  DMRG_REQUIRE(std::max(m, n) <= rw.len && r <= rw.r, "rsvd: work space too small");
+  auto tile = [&](int M, int N, int K) { return gemm_pick_tile(M, N, K, cx); };
+  // Range sketch of the orthonormal side: Y = A Om ('U' side) or A^H Om
+  fill_random<T>(st, rw.Om, (size_t)len_o * r, seed, 0, 1.0);
+  auto apply = [&](bool herm, const T* X, T* out) {  // out = A X or A^H X
+    if (!herm)
+      gemm<T>(st, OP_N, OP_N, m, r, n, T(1), A, lda, X, n, T(0), out, m, tile(m, r, n));
+    else
+      gemm<T>(st, OP_H, OP_N, n, r, m, T(1), A, lda, X, m, T(0), out, n, tile(n, r, m));
+  };
+  apply(scale_left, rw.Om, rw.Y);
+  orth_columns<T>(st, rw.Y, len_s, r, len_s, rw.C, rw.nrm);
+  for (int q = 0; q < power; ++q) {  // subspace (power) iterations, re-orthonormalised
+    apply(!scale_left, rw.Y, rw.Z);
+    orth_columns<T>(st, rw.Z, len_o, r, len_o, rw.C, rw.nrm);
+    apply(scale_left, rw.Z, rw.Y);
+    orth_columns<T>(st, rw.Y, len_s, r, len_s, rw.C, rw.nrm);
+  }
+  // Project and split the small matrix exactly, then map its orthonormal
+  // factor back: U = Q U_B ('U' side) or V^H = V_B^H Q^H ('V' side).
+  SvdParams p = prm;
+  p.subspace = false;
+  SvdResult res;
+  if (!scale_left) {
+    // B (r x n) = Q^H A; B = U_B (S Vh): right gets S Vh directly
+    gemm<T>(st, OP_H, OP_N, r, n, m, T(1), rw.Y, m, A, lda, T(0), rw.B, r, tile(r, n, m));
+    res = svd_split<T>(st, w, rw.B, r, n, r, k, cutoff, rel_cutoff, false, rw.Wb, r, right,
+                       nullptr, p);
+    gemm<T>(st, OP_N, OP_N, m, res.nk, r, T(1), rw.Y, m, rw.Wb, r, T(0), left, ldl,
+            tile(m, res.nk, r));
+  } else {
+    // B (m x r) = A Q; B = (U S) V_B^H: left gets U S directly
+    gemm<T>(st, OP_N, OP_N, m, r, n, T(1), A, lda, rw.Y, n, T(0), rw.B, m, tile(m, r, n));
+    res = svd_split<T>(st, w, rw.B, m, r, m, k, cutoff, rel_cutoff, true, left, ldl, rw.Wb,
+                       nullptr, p);
+    // right (nk x n) = V_B^H (nk x r) Q^H (r x n)
+    gemm<T>(st, OP_N, OP_H, res.nk, n, r, T(1), rw.Wb, res.nk, rw.Y, n, T(0), right, res.nk,
+            tile(res.nk, n, r));
+  }
+  return res;
+}
+
 template struct SvdWork<double>;
 template struct SvdWork<cdouble>;
 template SvdResult svd_split<double>(hipStream_t, SvdWork<double>&, const double*, int, int, int,
@@ -1428,6 +1597,15 @@ template SvdResult svd_split<double>(hipStream_t, SvdWork<double>&, const double
 template SvdResult svd_split<cdouble>(hipStream_t, SvdWork<cdouble>&, const cdouble*, int, int,
                                       int, int, double, double, bool, cdouble*, int, cdouble*,
                                       double*, const SvdParams&, const cdouble*, cdouble*);
+template struct RsvdWork<double>;
+template struct RsvdWork<cdouble>;
+template SvdResult rsvd_split<double>(hipStream_t, SvdWork<double>&, RsvdWork<double>&,
+                                      const double*, int, int, int, int, double, double, bool,
+                                      double*, int, double*, const SvdParams&, int, int, uint64_t);
+template SvdResult rsvd_split<cdouble>(hipStream_t, SvdWork<cdouble>&, RsvdWork<cdouble>&,
+                                       const cdouble*, int, int, int, int, double, double, bool,
+                                       cdouble*, int, cdouble*, const SvdParams&, int, int,
+                                       uint64_t);
 template int heig<double>(hipStream_t, SvdWork<double>&, const double*, int, int, double*, double*,
                           int, const SvdParams&);
 template int heig<cdouble>(hipStream_t, SvdWork<cdouble>&, const cdouble*, int, int, double*,

@@ -103,6 +103,35 @@ SvdResult svd_split(hipStream_t st, SvdWork<T>& w, const T* A, int m, int n, int
                     double* sigma_out, const SvdParams& prm, const T* warm = nullptr,
                     T* basis_out = nullptr);
 
+// Work space of rsvd_split: sketch, bases and the projected matrix, each
+// cap_len x cap_r, plus Gram-Schmidt scratch.
+template <class T>
+struct RsvdWork {
+  T *Om = nullptr, *Y = nullptr, *Z = nullptr, *B = nullptr, *Wb = nullptr, *C = nullptr;
+  double* nrm = nullptr;
+  int len = 0, r = 0;
+  void alloc(int cap_len, int cap_r);
+  void release();
+};
+
+// Randomised truncated split (the reference's RSVD option,
+// gpu-rocm/dmrg2-gpu/src/dmrg2_gpu_impl.h:1186-1266, Halko-Martinsson-Tropp):
+// a sketch of r = min(k + oversample, m, n) random combinations of the side
+// whose factor is orthonormal (uniform [-1, 1) entries from the splitmix64
+// stream `seed`), `power` re-orthonormalised subspace iterations, an
+// orthonormal basis Q by block Gram-Schmidt, and the exact split (svd_split,
+// every pair converged) of the r-dimensional projection; the orthonormal
+// factor is mapped back through Q.  Same outputs and truncation rule as
+// svd_split (no sigma_out, no warm start).  The result approximates the
+// exact split to within the sketch error: exact when the rank of A is at
+// most r, otherwise the kept space is off by roughly
+// (sigma_r / sigma_k)^(2 power + 1).
+template <class T>
+SvdResult rsvd_split(hipStream_t st, SvdWork<T>& w, RsvdWork<T>& rw, const T* A, int m, int n,
+                     int lda, int kmax, double cutoff, double rel_cutoff, bool scale_left, T* left,
+                     int ldl, T* right, const SvdParams& prm, int oversample, int power,
+                     uint64_t seed);
+
 // Eigen-decomposition of a Hermitian matrix H (N x N, ldh): eigenvalues
 // descending into evals (device, N), eigenvectors into evecs (N x N, lde).
 template <class T>

@@ -1,5 +1,6 @@
 // Warm start of the truncation SVD (see warm.hpp).
 #include "warm.hpp"
+#include "contract.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -72,7 +73,7 @@ template <class T>
 bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
                    double* hp, double start_tol, T** Zout) {
   const bool cx = is_cplx<T>::value;
-  DMRG_HIP(hipMemcpyAsync(Z0, Tin, (size_t)n * n * sizeof(T), hipMemcpyDeviceToDevice, st));
+  copy_dev<T>(st, Z0, Tin, (size_t)n * n);
   T* Z = Z0;
   T* Zn = Z1;
   // dev[it*kPolarParts + blk]: workgroup maxima of |Z_it^H Z_it - I|; the host

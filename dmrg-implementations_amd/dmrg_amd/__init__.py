@@ -31,10 +31,10 @@ PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 EXPORTED = [
     "dmrg_default_options", "dmrg_create", "dmrg_destroy", "dmrg_set_mpo",
     "dmrg_init_mps_random", "dmrg_init_mps_product", "dmrg_init_mps_neel", "dmrg_set_mps",
-    "dmrg_run", "dmrg_run_batch", "dmrg_get_energy", "dmrg_get_sweep_energies", "dmrg_get_bond_dims",
+    "dmrg_run", "dmrg_run_batch", "dmrg_run_pdmrg", "dmrg_get_energy", "dmrg_get_sweep_energies", "dmrg_get_bond_dims",
     "dmrg_get_mps", "dmrg_get_env", "dmrg_expectation", "dmrg_get_stats", "dmrg_reset_stats", "dmrg_last_error",
     "dmrg_ctx_create", "dmrg_ctx_destroy", "dmrg_op_heff2", "dmrg_op_heff2_f32", "dmrg_op_heff1", "dmrg_op_env_left",
-    "dmrg_op_env_left_noalloc", "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
+    "dmrg_op_env_left_noalloc", "dmrg_op_env_right", "dmrg_op_svd_split", "dmrg_op_rsvd_split", "dmrg_op_heig", "dmrg_op_tridiag_lowest", "dmrg_op_lanczos2",
     "dmrg_op_gemm", "dmrg_op_gemm_f32", "dmrg_dev_malloc", "dmrg_dev_free", "dmrg_memcpy_h2d", "dmrg_memcpy_d2h", "dmrg_memcpy_d2d",
     "dmrg_dev_synchronize", "dmrg_device_count",
 ]
@@ -52,7 +52,10 @@ class Options(C.Structure):
                 ("precision", C.c_int), ("profile", C.c_int), ("device", C.c_int),
                 ("svd_max_sweeps", C.c_int), ("svd_tol", C.c_double),
                 ("svd_warm", C.c_int), ("fp32_eig_rel", C.c_double),
-                ("svd_subspace", C.c_int), ("svd_tol_mixed", C.c_double)]
+                ("svd_subspace", C.c_int), ("svd_tol_mixed", C.c_double),
+                ("env_precision", C.c_int), ("svd_rsvd", C.c_int),
+                ("rsvd_oversample", C.c_int), ("rsvd_power", C.c_int), ("graph", C.c_int),
+                ("autotune", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -63,7 +66,10 @@ class Stats(C.Structure):
                 ("svd_sweeps", C.c_longlong), ("sweeps_done", C.c_int),
                 ("max_discarded", C.c_double), ("env_build_s", C.c_double),
                 ("sweeps_s_total", C.c_double), ("svd_warm_starts", C.c_longlong),
-                ("svd_bip_sweeps", C.c_longlong), ("svd_oa_steps", C.c_longlong)]
+                ("svd_bip_sweeps", C.c_longlong), ("svd_oa_steps", C.c_longlong),
+                ("trunc_sum_last", C.c_double), ("rsvd_calls", C.c_longlong),
+                ("graph_launches", C.c_longlong), ("graph_captures", C.c_longlong),
+                ("tuned_shapes", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -92,6 +98,7 @@ def lib():
         "dmrg_set_mps": (i, [vp, i, vp, i, i]),
         "dmrg_run": (i, [vp, i, i, C.POINTER(d)]),
         "dmrg_run_batch": (i, [vp, i, i, i, vp, vp]),
+        "dmrg_run_pdmrg": (i, [vp, i, i, i, i, i, i, C.POINTER(d)]),
         "dmrg_get_energy": (i, [vp, C.POINTER(d)]),
         "dmrg_get_sweep_energies": (i, [vp, C.POINTER(d), i, C.POINTER(i)]),
         "dmrg_get_bond_dims": (i, [vp, C.POINTER(i)]),
@@ -110,6 +117,8 @@ def lib():
         "dmrg_op_env_left_noalloc": (i, [vp, i, i, i, i, i, vp, vp, vp, vp]),
         "dmrg_op_env_right": (i, [vp, vp, i, i, i, i, i, vp, vp, vp, vp]),
         "dmrg_op_svd_split": (i, [vp, vp, i, i, i, i, d, d, i, vp, vp, vp, vp, C.POINTER(i)]),
+        "dmrg_op_rsvd_split": (i, [vp, vp, i, i, i, i, i, i, u64, d, d, i, vp, vp, vp,
+                                   C.POINTER(i)]),
         "dmrg_op_heig": (i, [vp, vp, i, i, vp, vp, vp, C.POINTER(i)]),
         "dmrg_op_tridiag_lowest": (i, [vp, i, C.POINTER(d), C.POINTER(d), C.POINTER(d),
                                        C.POINTER(d)]),
@@ -252,6 +261,13 @@ class DMRG:
     def run(self, n_sweeps, quiet=True):
         e = C.c_double(0)
         _check(lib().dmrg_run(self._h, n_sweeps, 1 if quiet else 0, C.byref(e)))
+        return e.value
+
+    def run_pdmrg(self, segments=2, outer=10, local=2, warmup=1, polish=1, quiet=True):
+        """Segment-parallel two-site sweep (dmrg_run_pdmrg)."""
+        e = C.c_double(0)
+        _check(lib().dmrg_run_pdmrg(self._h, segments, outer, local, warmup, polish,
+                                    1 if quiet else 0, C.byref(e)))
         return e.value
 
     def get_energy(self):
@@ -451,6 +467,26 @@ class Ops:
         lf = left.numpy().ravel(order="F")[: m * nk].reshape(m, nk, order="F")
         rf = right.numpy().ravel(order="F")[: nk * n].reshape(nk, n, order="F")
         return lf, rf, sig.numpy()[:nk], nk
+
+    def rsvd_split(self, M, k, oversample=10, power=0, seed=0x5EED0000, cutoff=1e-14,
+                   rel_cutoff=0.0, scale_left=False):
+        """Randomised truncated split of a device matrix M (m x n); returns
+        numpy (left, right, nk)."""
+        if len(M.shape) != 2:
+            raise ValueError("rsvd_split: M must be 2-D, got shape %s" % (M.shape,))
+        m, n = M.shape
+        dt = M.dtype
+        dtype_code(dt)
+        left = DeviceArray((m, k), dt)
+        right = DeviceArray((k, n), dt)
+        nk = C.c_int(0)
+        _check(lib().dmrg_op_rsvd_split(self._h, None, self._dt(M), m, n, k, oversample, power,
+                                        seed, cutoff, rel_cutoff, 1 if scale_left else 0, M.ptr,
+                                        left.ptr, right.ptr, C.byref(nk)))
+        nk = nk.value
+        lf = left.numpy().ravel(order="F")[: m * nk].reshape(m, nk, order="F")
+        rf = right.numpy().ravel(order="F")[: nk * n].reshape(nk, n, order="F")
+        return lf, rf, nk
 
     def heig(self, H):
         """Hermitian eigen-decomposition of a device matrix H (N x N); returns

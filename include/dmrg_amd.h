@@ -43,10 +43,11 @@ enum {
 
 enum { DMRG_DTYPE_F64 = 0, DMRG_DTYPE_C128 = 1 };
 enum { DMRG_SINGLE_SITE = 1, DMRG_TWO_SITE = 2 };
-/* Arithmetic of the H_eff matvec.  Environments, Lanczos (tridiagonal solve,
- * reorthogonalisation) and the truncation SVD are FP64 in every mode.
- * FP32: the matvec GEMMs on FP32 MFMA.  BF16X3: the same FP32 matvec with its
- * two GEMMs as split-BF16 products (each operand hi+mid+lo in bf16, the six
+/* Arithmetic of the H_eff matvec and of the environment updates.  Lanczos
+ * (tridiagonal solve, reorthogonalisation) and the truncation SVD are FP64 in
+ * every mode.  FP32: the matvec GEMMs, and (option env_precision = 1) the
+ * environment updates, on FP32 MFMA.  BF16X3: the same with the matvec's two
+ * GEMMs as split-BF16 products (each operand hi+mid+lo in bf16, the six
  * leading cross products on the bf16 MFMA, FP32 accumulation).  Both mixed
  * modes are real two-site only. */
 enum { DMRG_PREC_FP64 = 0, DMRG_PREC_FP32 = 1, DMRG_PREC_BF16X3 = 2 };
@@ -79,6 +80,36 @@ typedef struct dmrg_options {
                             which couples kept and discarded directions near
                             the cut at ~1e-13 |rho|_F; converging further would
                             resolve that rounding.                            */
+  int env_precision;     /* FP32 / BF16X3 modes only.  0 (default): FP64
+                            environments, converted to FP32 per bond; the
+                            energy is the FP64 Rayleigh quotient of the last
+                            bond.  1: the left and right environment updates
+                            run on FP32 MFMA and the environments stay FP32
+                            (each with an FP64 scale); the energy of every
+                            sweep is the FP64 <psi|H|psi>/<psi|psi> of the
+                            state.  FP32 environments accumulate rounding
+                            along the chain: |dE|/|E| 2.4e-10 against FP64 at
+                            L=128, chi=512 (2e-12 with FP64 environments).    */
+  int svd_rsvd;          /* 1: randomised two-site split (the reference's
+                            DMRG_GPU_OPT_RSVD, dmrg2_gpu_impl.h:1186-1266) on the
+                            bonds where the reference uses it (it keeps fewer
+                            than min(m, n) - rsvd_oversample values and
+                            m > 2 k); 0 (default): exact split everywhere.   */
+  int rsvd_oversample;   /* extra sketch columns, 10 as in the reference      */
+  int rsvd_power;        /* re-orthonormalised subspace iterations (0: the
+                            reference's single pass)                          */
+  int graph;             /* 1: each bond's Lanczos iterations between two
+                            convergence checks run as a captured HIP graph,
+                            cached per bond and bond dimensions and replayed
+                            (the reference's per-bond graph cache,
+                            dmrg2_gpu_impl.h:615-637); ignored with profile. */
+  int autotune;          /* 1: turn on the process-wide GEMM tile autotuner:
+                            the first call of each GEMM shape times the 32x32
+                            and 64x64 MFMA tiles (four or eight waves each)
+                            and keeps the fastest for every later call of
+                            that shape in the process.
+                            No reference counterpart (rocBLAS picks its own
+                            kernels).                                         */
 } dmrg_options;
 
 typedef struct dmrg_stats {
@@ -100,6 +131,12 @@ typedef struct dmrg_stats {
   long long svd_warm_starts; /* splits that started from a previous sweep's basis */
   long long svd_bip_sweeps;  /* Jacobi sweeps that rotated only kept-vs-discarded pairs */
   long long svd_oa_steps;    /* first-order all-pairs steps before the Jacobi sweeps */
+  double trunc_sum_last;     /* truncation error of the last sweep: the discarded
+                                weight sum(sigma_i^2) summed over its splits   */
+  long long rsvd_calls;      /* splits that took the randomised path            */
+  long long graph_launches;  /* Lanczos stretches replayed as graphs            */
+  long long graph_captures;  /* graphs captured (first use per bond and shape)  */
+  long long tuned_shapes;    /* GEMM shapes timed by the autotuner (process-wide) */
 } dmrg_stats;
 
 typedef struct dmrg_engine* dmrg_handle;
@@ -156,6 +193,20 @@ int dmrg_run(dmrg_handle h, int n_sweeps, int quiet, double* energy);
  * error stops the remaining chains and is returned. */
 int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* energies,
                    double* seconds);
+
+/* Segment-parallel two-site sweep on one GPU: replaces PDMRGGPU::run
+ * (gpu-rocm/pdmrg-gpu/src/pdmrg_gpu_impl.h:2585-2753; class at
+ * pdmrg_gpu.h:13-60).  `warmup` serial full sweeps, then up to `outer`
+ * iterations of `local` staggered passes: the chain's `segments` segments
+ * (lengths differing by at most one, at least 2 sites each) sweep their own
+ * bonds concurrently, even segments left to right and odd ones right to left
+ * and then the reverse, each pass followed by the boundary merges
+ * theta = psi_L diag(V) psi_R with V = 1/max(S, 1e-12) of the boundary's last
+ * split; stops when the energy changes by less than tol.  Then `polish`
+ * serial full sweeps.  Two-site FP64 / complex128 only (BAD_ARG otherwise).
+ * The energy is the last merge's, or the last polish sweep's. */
+int dmrg_run_pdmrg(dmrg_handle h, int segments, int outer, int local, int warmup, int polish,
+                   int quiet, double* energy);
 
 /* Replaces get_energy() (dmrg2_gpu.h:40). */
 int dmrg_get_energy(dmrg_handle h, double* energy);
@@ -247,6 +298,16 @@ int dmrg_op_env_right(dmrg_ctx_handle c, void* stream, int dtype, int ci, int co
 int dmrg_op_svd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, int k,
                       double cutoff, double rel_cutoff, int scale_left, const void* M,
                       void* left, void* right, double* sigma, int* nk_out);
+/* Randomised truncated split (the reference's RSVD path,
+ * dmrg2_gpu_impl.h:1186-1266): a sketch of r = min(k + oversample, m, n)
+ * uniform random combinations (splitmix64 stream `seed`), `power`
+ * re-orthonormalised subspace iterations, a block Gram-Schmidt basis Q and
+ * the exact split of the projected r-dimensional matrix.  Outputs as
+ * dmrg_op_svd_split (without sigma); exact when rank(M) <= r. */
+int dmrg_op_rsvd_split(dmrg_ctx_handle c, void* stream, int dtype, int m, int n, int k,
+                       int oversample, int power, uint64_t seed, double cutoff,
+                       double rel_cutoff, int scale_left, const void* M, void* left,
+                       void* right, int* nk_out);
 /* Hermitian eigen-decomposition H = V diag(evals) V^H of a device matrix
  * H (N x N, column-major, N <= 2048) by the block-Jacobi solver behind
  * dmrg_op_svd_split; evals descending (device, N), evecs (device, N x N).

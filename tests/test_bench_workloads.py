@@ -86,3 +86,16 @@ def test_profile_summaries_name_their_commands():
         text = open(f).read()
         assert "rocprofv3" in text and "bench.py" in text, f
         assert "_work" not in text, f
+
+
+def test_cpu_baseline_bond_weighting():
+    """The CPU baseline scales two timed centre bonds to a sweep by the
+    bonds' H_eff flops: at C4 (L=128, chi=512) the 2*127 updates count as
+    fewer centre-bond equivalents than 254 (the edge bonds are smaller) but
+    more than the 2*110 full-size interior ones; a chain of equal bonds
+    counts exactly 2(L-1)."""
+    wl = bench.WORKLOADS["c4"]
+    eq = bench.sweep_bond_equivalents(wl, wl["L"] // 2 - 1, 2)
+    assert 2 * 110 < eq < 2 * 127
+    flat = dict(wl, chi=1)
+    assert abs(bench.sweep_bond_equivalents(flat, 60, 2) - 2 * (wl["L"] - 1)) < 1e-9

@@ -255,6 +255,26 @@ def test_c3_workload_consistent(dmrg_amd):
     assert out[wl["chi"]] < out[64] + 1e-10
 
 
+def test_c3_rsvd_matches_exact_split(dmrg_amd):
+    """C3 (complex Josephson L=32, d=5, chi=256) with the randomised split on
+    the bonds where the reference would use it (m = 5 chi > 2 chi), two
+    subspace iterations: the sweep energies equal the exact split's to
+    1e-10 per site after 8 sweeps from the same seeded start."""
+    wl = bench.WORKLOADS["c3"]
+    Ws = bench.workload_mpo(wl)
+    E = {}
+    for rs in (0, 1):
+        eng = dmrg_amd.DMRG(wl["L"], 5, wl["chi"], 4, dtype="c128", tol=0.0,
+                            krylov_max=wl["krylov"], svd_rsvd=rs, rsvd_power=2)
+        eng.set_mpo(Ws)
+        eng.initialize_mps_random(seed=wl["seed"], scale=0.1)
+        E[rs] = eng.run(8)
+        if rs:
+            assert eng.stats()["rsvd_calls"] > 0
+        eng.close()
+    assert abs(E[1] - E[0]) < 1e-10 * wl["L"], E
+
+
 @pytest.fixture(scope="module")
 def c4_fp64(dmrg_amd):
     wl = bench.WORKLOADS["c4"]
@@ -275,6 +295,20 @@ def test_c4_mixed_matches_fp64(dmrg_amd, c4_fp64, prec):
 
 
 C5_SWEEPS = 6
+
+
+def test_c4_fp32_environments(dmrg_amd, c4_fp64):
+    """C4 with the environment updates on FP32 MFMA as well (env_precision =
+    1): the FP32 rounding that the environments accumulate along the chain
+    bounds the agreement with the FP64 run at |dE|/|E| <= 1e-9 (measured
+    2.4e-10; 2e-12 with FP64 environments, the default).  The energy is the
+    FP64 <psi|H|psi> of the normalised state."""
+    wl = bench.WORKLOADS["c4"]
+    E64 = c4_fp64[0]
+    E, e, n2 = _heisenberg_run(dmrg_amd, wl["L"], wl["chi"], 10, wl["seed"], precision=1,
+                               env_precision=1)
+    assert abs(E - E64) / abs(E64) <= 1e-9, (E, E64)
+    assert abs(n2 - 1) < 1e-9 and abs(e - E) < 1e-9 * wl["L"]
 
 
 @pytest.fixture(scope="module")

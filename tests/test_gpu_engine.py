@@ -56,17 +56,20 @@ def test_two_site_matches_oracle_sweep_by_sweep(dmrg_amd):
     eng.close()
 
 
+@pytest.mark.parametrize("env", [0, 1], ids=["env64", "env32"])
 @pytest.mark.parametrize("mixed", [1, 2])
-def test_fp32_matvec_mode_tracks_fp64(dmrg_amd, mixed):
+def test_fp32_matvec_mode_tracks_fp64(dmrg_amd, mixed, env):
     """FP32 H_eff matvecs (FP32 MFMA, or split-BF16 products) with FP64
-    Lanczos/SVD/environments: the reported energy (FP64 Rayleigh quotient at
-    the last bond) stays within 1e-9 per site of the FP64 run from the same
-    seeded state."""
+    Lanczos/SVD, and FP64 environments (env64) or FP32 environment updates
+    (env32): the reported energy (the FP64 Rayleigh quotient at the last
+    bond, or the FP64 <psi|H|psi> of the state) stays within 1e-9 per site of
+    the FP64 run from the same seeded state."""
     L, chi = 16, 32
     Ws = models.heisenberg_mpo(L)
     out = {}
     for prec in (0, mixed):
-        eng = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20, precision=prec)
+        eng = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20, precision=prec,
+                            env_precision=env)
         eng.set_mpo(Ws)
         eng.initialize_mps_random(seed=7, scale=0.1)
         out[prec] = eng.run(6)
@@ -75,6 +78,37 @@ def test_fp32_matvec_mode_tracks_fp64(dmrg_amd, mixed):
         assert abs(e - out[prec]) < 1e-9 * L
         eng.close()
     assert abs(out[mixed] - out[0]) < 1e-9 * L, out
+
+
+def test_fp32_environments_match_fp64(dmrg_amd):
+    """The FP32 environment chain of the mixed modes (env_precision = 1):
+    for the same MPS (a converged FP64 state of a Heisenberg L=24 chain, and
+    its unnormalised seeded start), every right environment built on FP32
+    MFMA, times its FP64 scale, equals the FP64 engine's to FP32 accuracy
+    accumulated over the chain: |R32 - R64| <= 1e-5 max|R64| per
+    environment.  A dropped scale, a wrong MPO copy or a transposed operand
+    would leave O(1) differences."""
+    L, chi = 24, 48
+    Ws = models.heisenberg_mpo(L)
+    for sweeps in (0, 4):
+        ref = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20)
+        ref.set_mpo(Ws)
+        ref.initialize_mps_random(seed=24, scale=0.1)
+        ref.run(sweeps)
+        ref.run(0)  # rebuild the right environments of the final state
+        mps = ref.get_mps()
+        mix = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20, precision=1, env_precision=1)
+        mix.set_mpo(Ws)
+        mix.set_mps(mps)
+        mix.run(0)
+        for i in range(L + 1):
+            r64 = ref.get_env(False, i)
+            r32 = mix.get_env(False, i)
+            assert r32.shape == r64.shape
+            scale = np.max(np.abs(r64))
+            assert np.max(np.abs(r32 - r64)) <= 1e-5 * scale, (sweeps, i)
+        ref.close()
+        mix.close()
 
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
@@ -329,3 +363,109 @@ def test_rejected_set_mpo_leaves_no_mpo(dmrg_amd):
     E = eng.run(8)
     assert abs(E - models.ground_energy(models.dense_heisenberg(L))) < 1e-9
     eng.close()
+
+
+@pytest.mark.parametrize("power", [0, 2])
+def test_rsvd_engine_josephson(dmrg_amd, power):
+    """The engine with the randomised split (option svd_rsvd) on a complex
+    Josephson chain (L=12, d=5, chi=24: m = 5 chi > 2 chi on the interior
+    bonds, so the reference's RSVD condition holds there): it runs the
+    randomised path, and the converged energy agrees with the exact split's
+    -- within 1e-6 per site for the reference's single pass, 1e-9 with two
+    subspace iterations."""
+    L, chi = 12, 24
+    Ws = models.josephson_mpo(L, n_max=2, E_J=2.0, E_C=1.0, phi_ext=np.pi / 3)
+    out = {}
+    for rs in (0, 1):
+        eng = dmrg_amd.DMRG(L, 5, chi, 4, dtype="c128", tol=0.0, krylov_max=20, svd_rsvd=rs,
+                            rsvd_power=power)
+        eng.set_mpo(Ws)
+        eng.initialize_mps_random(seed=12, scale=0.1)
+        out[rs] = eng.run(6)
+        if rs:
+            assert eng.stats()["rsvd_calls"] > 0
+        e, n2 = eng.expectation()
+        assert abs(n2 - 1) < 1e-10 and abs(e - out[rs]) < 1e-10 * L
+        eng.close()
+    tol = 1e-6 if power == 0 else 1e-9
+    assert abs(out[1] - out[0]) < tol * L, out
+
+
+@pytest.mark.parametrize("variant,prec", [(2, 0), (2, 1), (1, 0)], ids=["two_fp64", "two_fp32",
+                                                                        "one_fp64"])
+def test_graph_replay_matches_eager(dmrg_amd, variant, prec):
+    """Lanczos stretches replayed as captured HIP graphs (option graph):
+    every sweep energy equals the eager run's bit for bit (the same kernels
+    in the same order), across sweeps in which each bond's graphs are
+    replayed again between eager splits and environment updates."""
+    L, chi = 14, 24
+    Ws = models.heisenberg_mpo(L)
+    res = {}
+    for g in (0, 1):
+        eng = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20, variant=variant,
+                            precision=prec, graph=g)
+        eng.set_mpo(Ws)
+        eng.initialize_mps_random(seed=14, scale=0.1)
+        eng.run(5)
+        res[g] = eng.sweep_energies()
+        st = eng.stats()
+        if g:
+            assert st["graph_launches"] > st["graph_captures"] > 0
+        else:
+            assert st["graph_launches"] == 0
+        eng.close()
+    assert np.array_equal(res[0], res[1]), (res[0], res[1])
+
+
+@pytest.mark.parametrize("segments,cplx", [(2, False), (4, False), (3, True)],
+                         ids=["p2", "p4", "p3_complex"])
+def test_pdmrg_matches_serial(dmrg_amd, segments, cplx):
+    """Segment-parallel sweep (the reference's pdmrg-gpu) from a seeded start:
+    one serial warm-up sweep, staggered segment sweeps with boundary merges
+    (V = 1/S), and one serial polish sweep reach the serial two-site
+    engine's converged energy to 1e-8 per site; the returned state is
+    normalised and <psi|H|psi> equals the returned energy.  Heisenberg L=24
+    (chi=32) in real arithmetic, the Josephson chain L=12 (d=5, chi=24) in
+    complex."""
+    if cplx:
+        L, d, chi, D = 12, 5, 24, 4
+        Ws = models.josephson_mpo(L, n_max=2, E_J=2.0, E_C=1.0, phi_ext=np.pi / 3)
+        dt = "c128"
+    else:
+        L, d, chi, D = 24, 2, 32, 5
+        Ws = models.heisenberg_mpo(L)
+        dt = "f64"
+    ref = dmrg_amd.DMRG(L, d, chi, D, dtype=dt, tol=1e-12, krylov_max=20)
+    ref.set_mpo(Ws)
+    ref.initialize_mps_random(seed=L, scale=0.1)
+    E_ref = ref.run(20)
+    ref.close()
+    eng = dmrg_amd.DMRG(L, d, chi, D, dtype=dt, tol=1e-11, krylov_max=20)
+    eng.set_mpo(Ws)
+    eng.initialize_mps_random(seed=L, scale=0.1)
+    E = eng.run_pdmrg(segments=segments, outer=20, local=2, warmup=1, polish=2)
+    st = eng.stats()
+    assert st["lanczos_calls"] > 0 and st["svd_calls"] > 0
+    e, n2 = eng.expectation()
+    eng.close()
+    assert abs(n2 - 1) < 1e-10 and abs(e - E) < 1e-10 * L
+    assert abs(E - E_ref) < 1e-8 * L, (E, E_ref)
+
+
+def test_pdmrg_rejects_bad_arguments(dmrg_amd):
+    """One segment, segments shorter than 2 sites, and the mixed modes are
+    refused before any work (BAD_ARG)."""
+    L = 8
+    eng = dmrg_amd.DMRG(L, 2, 16, 5, tol=0.0, krylov_max=10)
+    eng.set_mpo(models.heisenberg_mpo(L))
+    eng.initialize_mps_random(seed=1, scale=0.1)
+    for seg in (1, 5):
+        with pytest.raises(dmrg_amd.DMRGError):
+            eng.run_pdmrg(segments=seg)
+    eng.close()
+    mix = dmrg_amd.DMRG(L, 2, 16, 5, tol=0.0, krylov_max=10, precision=1)
+    mix.set_mpo(models.heisenberg_mpo(L))
+    mix.initialize_mps_random(seed=1, scale=0.1)
+    with pytest.raises(dmrg_amd.DMRGError):
+        mix.run_pdmrg(segments=2)
+    mix.close()

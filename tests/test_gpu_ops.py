@@ -247,3 +247,93 @@ def test_lanczos2_matches_oracle(dmrg_amd, ops, cplx):
     psi = dth.numpy()
     ov = abs(np.vdot(psi_ref.ravel(), psi.ravel()))
     assert abs(ov - 1) < 1e-9
+
+
+def _low_rank(rng, m, n, rank, cplx, decay):
+    """m x n matrix of exact rank `rank` with singular values e^{-j/decay}."""
+    def orth(k, r):
+        q, _ = np.linalg.qr(rand(rng, (k, r), cplx))
+        return q
+    s = np.exp(-np.arange(rank) / decay)
+    return (orth(m, rank) * s) @ orth(n, rank).conj().T, s
+
+
+@pytest.mark.parametrize("scale_left", [False, True], ids=["R", "L"])
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
+def test_rsvd_split_exact_when_rank_fits_sketch(dmrg_amd, ops, cplx, scale_left):
+    """The randomised split (sketch r = k + 10) of a matrix of rank k + 5 <= r:
+    the sketch spans the whole range, so the result equals the exact
+    truncated split -- kept product equal to LAPACK's best rank-k
+    approximation and the orthonormal factor orthonormal.  m > 2k as on the
+    bonds where the engine uses it."""
+    m, n, k = 320, 240, 40
+    rng = np.random.default_rng(11 + cplx + 2 * scale_left)
+    M, s = _low_rank(rng, m, n, k + 5, cplx, 12.0)
+    left, right, nk = ops.rsvd_split(dmrg_amd.DeviceArray.from_numpy(M), k, oversample=10,
+                                     power=0, seed=77, scale_left=scale_left)
+    assert nk == k
+    U, S, Vh = np.linalg.svd(M)
+    best = (U[:, :k] * S[:k]) @ Vh[:k]
+    assert rel_err(left @ right, best) < 1e-10
+    q = left if not scale_left else right.conj().T
+    assert np.linalg.norm(q.conj().T @ q - np.eye(nk)) < 1e-12 * nk
+
+
+@pytest.mark.parametrize("power,bound", [(0, 0.5), (2, 1e-5)])
+def test_rsvd_split_graded_spectrum(dmrg_amd, ops, power, bound):
+    """Full-rank graded spectrum (s_j = e^{-j/8}, m = 400, n = 300, k = 40):
+    the randomised split's truncation residual |M - left right| exceeds the
+    optimal sqrt(sum_{j>=k} s_j^2) by at most `bound` of it (two subspace
+    iterations measured 2.3e-6), and the orthonormal factor stays
+    orthonormal."""
+    m, n, k = 400, 300, 40
+    rng = np.random.default_rng(5)
+    M, s = _low_rank(rng, m, n, n, False, 8.0)
+    left, right, nk = ops.rsvd_split(dmrg_amd.DeviceArray.from_numpy(M), k, oversample=10,
+                                     power=power, seed=5)
+    assert nk == k
+    opt = np.sqrt(np.sum(s[k:] ** 2))
+    resid = np.linalg.norm(M - left @ right)
+    assert opt <= resid * (1 + 1e-12)
+    assert resid <= opt * (1 + bound), (resid / opt - 1)
+    assert np.linalg.norm(left.T @ left - np.eye(nk)) < 1e-12 * nk
+
+
+def test_rsvd_split_rank_deficient(dmrg_amd, ops):
+    """Rank 12 < k = 20 < r: the dependent sketch columns are dropped and the
+    cutoff keeps exactly the 12 nonzero singular values; the product
+    reproduces M."""
+    m, n, k = 200, 160, 20
+    rng = np.random.default_rng(3)
+    M, s = _low_rank(rng, m, n, 12, False, 4.0)
+    left, right, nk = ops.rsvd_split(dmrg_amd.DeviceArray.from_numpy(M), k, seed=9)
+    assert nk == 12
+    assert rel_err(left @ right, M) < 1e-12
+    assert np.linalg.norm(left.T @ left - np.eye(nk)) < 1e-12 * nk
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["fp32", "bf16x3"])
+@pytest.mark.parametrize("cL,cR", [(5, 7), (16, 24), (33, 40), (64, 64), (128, 96)])
+def test_heff2_f32_shapes(dmrg_amd, ops, prec, cL, cR):
+    """The mixed modes' two-site matvec (FP32 MFMA, or split-BF16 products on
+    operands split once into bf16 planes) at bond shapes whose sizes are and
+    are not multiples of 8 (16-byte plane vectors, or element-wise loads at
+    the tile edges): elementwise |out - ref| <= 5e-5 (|L| |theta| |WW| |R|)
+    against the float64 contraction of the same float32 inputs."""
+    d, D = 2, 5
+    rng = np.random.default_rng(cL * 131 + cR + prec)
+    f32 = lambda a: a.astype(np.float32)
+    L = f32(rng.uniform(-1, 1, (cL, D, cL)))
+    R = f32(rng.uniform(-1, 1, (cR, D, cR)))
+    th = f32(rng.uniform(-1, 1, (cL, d, d, cR)))
+    Ws = models.heisenberg_mpo(4)
+    W1, W2 = Ws[1], Ws[2]
+    ref = od.apply_heff2(L.astype(np.float64), R.astype(np.float64), W1, W2, th.astype(np.float64))
+    bound = od.apply_heff2(np.abs(L).astype(np.float64), np.abs(R).astype(np.float64), np.abs(W1),
+                           np.abs(W2), np.abs(th).astype(np.float64))
+    dev = dmrg_amd.DeviceArray.from_numpy
+    out = dmrg_amd.DeviceArray(th.shape, np.float32)
+    ops.heff2_f32(dev(L, np.float32), dev(R, np.float32), dev(W1), dev(W2), dev(th, np.float32),
+                  out, prec=prec)
+    err = np.max(np.abs(out.numpy().astype(np.float64) - ref) / bound)
+    assert err < 5e-5, err

@@ -270,3 +270,24 @@ def test_svd_split2_factors(direction):
     assert np.abs(Q.conj().T @ Q - np.eye(k)).max() < 1e-13
     assert abs(np.linalg.norm(M - A @ B) ** 2 - np.sum(s[k:] ** 2)) < 1e-12
     assert abs(disc - np.sum(s[k:] ** 2)) < 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_to_pdmrg_env_converts_engine_layout(cplx):
+    """The CPU baseline hands the engine's environments (ket, mpo, bra) to the
+    restated pdmrg step through to_pdmrg_env; the converted environment must
+    be the one the pdmrg update itself builds from the same tensors, pinned by
+    the reference einsum (test_numerics.py:162-192): a plain reshape, or a
+    transpose of the wrong axes, fails."""
+    rng = np.random.default_rng(9)
+    chi, chi2, D, d = 3, 4, 5, 2
+    Lp = _r(rng, (chi, D, chi), cplx)
+    A = _r(rng, (chi, d, chi2), cplx)
+    Wp = _r(rng, (D, D, d, d))
+    ref = np.einsum("awc,atp,wmst,csq->pmq", Lp, A.conj(), Wp, A, optimize=True)
+    eng = od.update_left_env(Lp.transpose(2, 1, 0), A, Wp.transpose(0, 2, 3, 1))
+    got = ps.to_pdmrg_env(eng)
+    assert got.flags["C_CONTIGUOUS"]
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(np.ascontiguousarray(eng).reshape(ref.shape) - ref).max() > 1e-3
+    assert np.abs(eng.transpose(0, 1, 2) - ref).max() > 1e-3
