@@ -542,23 +542,17 @@ namespace {
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 constexpr int BF_BK = 32, BF_LD = 40;
 
-__device__ inline unsigned short bf16_rne(float x) {
-  unsigned u;
-  memcpy(&u, &x, 4);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ inline float bf16_f(unsigned short h) {
-  const unsigned u = (unsigned)h << 16;
-  float x;
-  memcpy(&x, &u, 4);
-  return x;
-}
+// x = hi + mid + lo: the float -> bf16 casts compile to gfx950's
+// v_cvt_pk_bf16_f32 (round to nearest even, two values per instruction), the
+// bf16 -> float ones to a shift.
 __device__ inline void split3(float x, unsigned short& h, unsigned short& m, unsigned short& l) {
-  h = bf16_rne(x);
-  const float r1 = x - bf16_f(h);
-  m = bf16_rne(r1);
-  l = bf16_rne(r1 - bf16_f(m));
+  const __bf16 bh = (__bf16)x;
+  const float r1 = x - (float)bh;
+  const __bf16 bm = (__bf16)r1;
+  const __bf16 bl = (__bf16)(r1 - (float)bm);
+  memcpy(&h, &bh, 2);
+  memcpy(&m, &bm, 2);
+  memcpy(&l, &bl, 2);
 }
 
 template <bool KFA, bool KFB>

@@ -469,3 +469,22 @@ def test_pdmrg_rejects_bad_arguments(dmrg_amd):
     with pytest.raises(dmrg_amd.DMRGError):
         mix.run_pdmrg(segments=2)
     mix.close()
+
+
+def test_autotuned_tiles_match(dmrg_amd):
+    """With the GEMM tile autotuner on, every GEMM shape of the sweep is timed
+    once over the candidate tiles and the fastest kept; the sweep energies
+    agree with the heuristic tiles' to 1e-10 per site (tiles change only the
+    order of the floating-point sums)."""
+    L, chi = 20, 48
+    Ws = models.heisenberg_mpo(L)
+    E = {}
+    for at in (0, 1):
+        eng = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=20, autotune=at)
+        eng.set_mpo(Ws)
+        eng.initialize_mps_random(seed=20, scale=0.1)
+        E[at] = eng.sweep_energies() if eng.run(4) is None else eng.sweep_energies()
+        if at:
+            assert eng.stats()["tuned_shapes"] > 0
+        eng.close()
+    assert np.max(np.abs(np.asarray(E[1]) - np.asarray(E[0]))) < 1e-10 * L, E
