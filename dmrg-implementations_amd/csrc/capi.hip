@@ -154,7 +154,26 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
     std::mutex mu;
     int err_code = 0;
     std::string err_msg;
-    auto worker = [&]() {
+    const int nt = std::max(1, std::min(threads, n));
+    // One stream per worker, created here in worker order, carries both the
+    // bond work and the environment updates of the chains that worker runs:
+    // the runtime spreads streams over its hardware queues in creation order,
+    // so the workers share the queues evenly, and a chain never waits behind
+    // another chain's kernels on a queue that it shares with it more than
+    // the worker count forces.
+    struct Streams {
+      std::vector<hipStream_t> s;
+      ~Streams() {
+        for (hipStream_t x : s) (void)hipStreamDestroy(x);
+      }
+    } ws;
+    for (int t = 0; t < nt; ++t) {
+      hipStream_t x = nullptr;
+      DMRG_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+      ws.s.push_back(x);
+    }
+    auto worker = [&](int t) {
+      hipStream_t st = ws.s[t];
       for (;;) {
         if (stop.load()) return;
         const int j = next.fetch_add(1);
@@ -163,7 +182,18 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
         try {
           const auto t0 = std::chrono::steady_clock::now();
           double E = 0.0;
-          hs[i]->with([&](auto& e) { E = e.run(n_sweeps, true); });
+          hs[i]->with([&](auto& e) {
+            struct Lend {
+              decltype(e)& eng;
+              ~Lend() {
+                try { eng.restore_streams(); } catch (...) {}
+              }
+            };
+            e.borrow_streams(st, st);
+            Lend lend{e};
+            E = e.run(n_sweeps, true);
+            e.restore_streams();
+          });
           energies[i] = E;
           if (seconds)
             seconds[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -183,10 +213,9 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
         }
       }
     };
-    const int nt = std::max(1, std::min(threads, n));
     std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
-    worker();
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker, t);
+    worker(0);
     for (auto& th : pool) th.join();
     if (err_code) throw Error(err_code, err_msg);
   });

@@ -308,7 +308,38 @@ void Engine<T>::alloc_all() {
 }
 
 template <class T>
+void Engine<T>::borrow_streams(hipStream_t main, hipStream_t env) {
+  DMRG_REQUIRE(main && env, "borrow_streams: NULL stream");
+  DMRG_REQUIRE(!own_main, "borrow_streams: already borrowing");
+  DMRG_HIP(hipStreamSynchronize(s_main));
+  DMRG_HIP(hipStreamSynchronize(s_env));
+  own_main = s_main;
+  own_env = s_env;
+  s_main = main;
+  s_env = env;
+}
+
+template <class T>
+void Engine<T>::restore_streams() {
+  if (!own_main) return;
+  const hipError_t e1 = hipStreamSynchronize(s_main);
+  const hipError_t e2 = hipStreamSynchronize(s_env);
+  s_main = own_main;
+  s_env = own_env;
+  own_main = own_env = nullptr;
+  DMRG_HIP(e1);
+  DMRG_HIP(e2);
+}
+
+template <class T>
 void Engine<T>::free_all() {
+  if (own_main) {  // still borrowing: drain the lent streams, destroy only our own
+    (void)hipStreamSynchronize(s_main);
+    (void)hipStreamSynchronize(s_env);
+    s_main = own_main;
+    s_env = own_env;
+    own_main = own_env = nullptr;
+  }
   seg_free();
   if (s_main) (void)hipStreamSynchronize(s_main);
   if (s_env) (void)hipStreamSynchronize(s_env);

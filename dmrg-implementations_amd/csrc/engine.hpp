@@ -162,6 +162,13 @@ class Engine {
   void init_neel();
   void set_mps(int site, const T* A, int cL, int cR);
   double run(int n_sweeps, bool quiet);
+  // Run on the caller's stream(s) until restore_streams(): both of the
+  // engine's own streams are drained first, and `main` may equal `env` (then
+  // the environment updates queue behind the bond work).  The batch runner
+  // lends each worker thread's stream to the chains it runs, so a hardware
+  // queue carries whole chains rather than every chain's main or env stream.
+  void borrow_streams(hipStream_t main, hipStream_t env);
+  void restore_streams();
   // Segment-parallel sweep (PDMRG, the reference's pdmrg-gpu,
   // gpu-rocm/pdmrg-gpu/src/pdmrg_gpu_impl.h:2407-2753), two-site FP64.
   double run_pdmrg(const PdmrgParams& p, bool quiet);
@@ -239,6 +246,7 @@ class Engine {
   std::vector<int> pd_Vn;     // its length (the boundary bond dimension)
 
   hipStream_t s_main = nullptr, s_env = nullptr;
+  hipStream_t own_main = nullptr, own_env = nullptr;  // while borrowing: the engine's streams
   hipEvent_t ev_canon = nullptr, ev_env = nullptr;
   bool env_pending = false;
   bool mpo_set = false;

@@ -186,9 +186,11 @@ int dmrg_run(dmrg_handle h, int n_sweeps, int quiet, double* energy);
 /* Batched solve of independent chains (no reference counterpart; the
  * reference runs chains one binary invocation at a time).  hs[0..n) are
  * distinct engines (BAD_ARG if one is listed twice) created and set up as
- * usual (any mix of L, chi, d, D, variant and precision); each runs dmrg_run(n_sweeps, quiet) on its own streams, and
- * `threads` host threads feed them to the GPU concurrently, longest chain
- * first, so the latency-bound phases of different chains overlap.
+ * usual (any mix of L, chi, d, D, variant and precision); each runs
+ * dmrg_run(n_sweeps, quiet).  `threads` host threads feed them to the GPU
+ * concurrently, longest chain first, so the latency-bound phases of different
+ * chains overlap; each thread owns one stream, on which the chains it runs do
+ * all their work (the engines' own streams are idle meanwhile).
  * energies[i] (and seconds[i], may be NULL) get chain i's result.  The first
  * error stops the remaining chains and is returned. */
 int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* energies,
