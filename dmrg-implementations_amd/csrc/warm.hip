@@ -12,8 +12,65 @@ namespace {
 __device__ inline double absv(double a) { return fabs(a); }
 __device__ inline double absv(cdouble a) { return sqrt(a.x * a.x + a.y * a.y); }
 
-// part[blk] = max |P - I| over the workgroup's entries (NaN as +inf);
-// P <- (a/2)(3I - a^2 P).  Grid of kPolarParts workgroups.
+// Gershgorin bounds of the Hermitian P = Z^H Z, one wave per column j:
+// part[3 blk + 0] = min_j (P_jj - r_j), part[3 blk + 1] = max_j (P_jj + r_j),
+// r_j = sum_{i != j} |P_ij|, and part[3 blk + 2] = max |P - I| over the
+// workgroup's columns (NaN as +inf, and as -inf / +inf in the bounds).  Every
+// eigenvalue of P lies in [min, max].  Grid of kPolarParts workgroups.
+template <class T>
+__global__ __launch_bounds__(256) void k_gersh(const T* __restrict__ P, int n,
+                                               double* __restrict__ part) {
+  __shared__ double red[3][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double gmin = INFINITY, gmax = -INFINITY, dev = 0.0;
+  for (int j = blockIdx.x * 4 + wave; j < n; j += gridDim.x * 4) {
+    double off = 0.0, diag = 0.0, dv = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const T p = P[i + (size_t)j * n];
+      if (i == j) {
+        diag = real(p);
+        dv = fmax(dv, absv(p - T(1)));
+      } else {
+        const double v = absv(p);
+        off += v;
+        dv = fmax(dv, v);
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o);
+      diag += __shfl_xor(diag, o);
+      dv = fmax(dv, __shfl_xor(dv, o));
+    }
+    if (!(off == off) || !(diag == diag) || !(dv == dv)) {
+      gmin = -INFINITY;
+      gmax = INFINITY;
+      dev = INFINITY;
+    } else {
+      gmin = fmin(gmin, diag - off);
+      gmax = fmax(gmax, diag + off);
+      dev = fmax(dev, dv);
+    }
+  }
+  if (lane == 0) {
+    red[0][wave] = gmin;
+    red[1][wave] = gmax;
+    red[2][wave] = dev;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      gmin = fmin(gmin, red[0][w]);
+      gmax = fmax(gmax, red[1][w]);
+      dev = fmax(dev, red[2][w]);
+    }
+    part[3 * blockIdx.x] = gmin;
+    part[3 * blockIdx.x + 1] = gmax;
+    part[3 * blockIdx.x + 2] = dev;
+  }
+}
+
+// part[blk] = max |P - I| over the workgroup's entries (NaN as +inf; part may
+// be null); P <- (a/2)(3I - a^2 P).  Grid of kPolarParts workgroups.
 template <class T>
 __global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, double a,
                                                  double* __restrict__ part) {
@@ -30,6 +87,7 @@ __global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, doubl
     mx = fmax(mx, v);
     P[e] = (3.0 * one - (a * a) * p) * (0.5 * a);
   }
+  if (!part) return;
   red[t] = mx;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -37,6 +95,18 @@ __global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, doubl
     __syncthreads();
   }
   if (t == 0) part[blockIdx.x] = red[0];
+}
+
+// Steps of the scaled iteration from the lower bound l until the bound
+// reaches 1 to rounding (the iterate is then unitary in exact arithmetic).
+int polar_steps(double l) {
+  int k = 0;
+  while (l < 1.0 - 1e-15 && k < kPolarMaxIter) {
+    const double a = std::sqrt(3.0 / (1.0 + l + l * l));
+    l = std::min(1.0, 0.5 * a * l * (3.0 - a * a * l * l));
+    ++k;
+  }
+  return k;
 }
 
 }  // namespace
@@ -77,20 +147,50 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
   T* Z = Z0;
   T* Zn = Z1;
   // dev[it*kPolarParts + blk]: workgroup maxima of |Z_it^H Z_it - I|; the host
-  // reduces them after the first step and after steps 9, 11, 13, ... (from
-  // l = 3e-4 the scaled iteration needs about 13; steps past convergence keep
-  // Z unitary to rounding, the iteration being stable at its fixed point, and
-  // by then a = 1: the bound l reaches 1 no later than the singular values).
+  // reduces them after the first step and after steps 9, 11, 13, ... from the
+  // fixed bound (which needs 13), or K-2, K, K+2, ... from a Gershgorin bound
+  // that needs K steps to reach 1 (steps past convergence keep Z unitary
+  // to rounding, the iteration being stable at its fixed point, and by then
+  // a = 1: the bound l reaches 1 no later than the singular values).  The
+  // first step also bounds the spectrum of Z^H Z by Gershgorin's discs: when
+  // they leave out 0, sqrt(min) replaces the fixed start l = kPolarL0 (an
+  // overlap of two bases that differ little is close to unitary, and its
+  // iteration then needs 3-6 steps instead of ~13).
   int seen = 0;
   double l = kPolarL0;  // lower bound of the smallest singular value of Z
+  int first_check = 9;
   for (int it = 0; it < kPolarMaxIter; ++it) {
+    gemm_herm<T>(st, OP_H, OP_N, n, n, T(1), Z, n, Z, n, P, n, gemm_pick_tile(n, n, n, cx));
+    if (it == 0) {
+      double* const gp = hp + 2 * (size_t)kPolarParts;  // past slots 0 and 1 of the read-back
+      hipLaunchKernelGGL(k_gersh<T>, dim3(kPolarParts), dim3(256), 0, st, (const T*)P, n, dev);
+      DMRG_LAUNCH_CHECK();
+      DMRG_HIP(hipMemcpyAsync(gp, dev, 3 * (size_t)kPolarParts * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+      DMRG_HIP(hipStreamSynchronize(st));
+      double gmin = INFINITY, h = 0.0;
+      for (int b = 0; b < kPolarParts; ++b) {
+        gmin = std::min(gmin, gp[3 * b]);
+        const double v = gp[3 * b + 2];
+        h = (v == v) ? std::max(h, v) : INFINITY;
+      }
+      if (!(h == h) || h > start_tol) return false;
+      if (h <= 1e-14) {
+        *Zout = Z;
+        return true;
+      }
+      if (gmin == gmin && gmin > kPolarL0 * kPolarL0) {
+        l = std::min(1.0, std::sqrt(gmin));
+        first_check = std::max(1, polar_steps(l) - 2);
+      }
+      seen = 1;
+    }
     const double a = l < 1.0 ? std::sqrt(3.0 / (1.0 + l + l * l)) : 1.0;
     l = std::min(1.0, 0.5 * a * l * (3.0 - a * a * l * l));
-    gemm_herm<T>(st, OP_H, OP_N, n, n, T(1), Z, n, Z, n, P, n, gemm_pick_tile(n, n, n, cx));
     hipLaunchKernelGGL(k_ns_prep<T>, dim3(kPolarParts), dim3(256), 0, st, P, n, a,
-                       dev + (size_t)it * kPolarParts);
+                       it == 0 ? nullptr : dev + (size_t)it * kPolarParts);
     DMRG_LAUNCH_CHECK();
-    if (it == 0 || (it >= 9 && it % 2 == 1) || it == kPolarMaxIter - 1) {
+    if (it > 0 && ((it >= first_check && (it - first_check) % 2 == 0) || it == kPolarMaxIter - 1)) {
       DMRG_HIP(hipMemcpyAsync(hp + (size_t)seen * kPolarParts, dev + (size_t)seen * kPolarParts,
                               (size_t)(it + 1 - seen) * kPolarParts * sizeof(double),
                               hipMemcpyDeviceToHost, st));
@@ -104,7 +204,7 @@ bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, doub
         // a zero singular value never converges: give up when the start is
         // too far or the deviation is still O(1) after 24 steps (from
         // s = 1e-5 the scaled iteration needs about 24)
-        if (!(h == h) || (j == 0 && h > start_tol) || (j >= 24 && h > 0.5)) return false;
+        if (!(h == h) || (j >= 24 && h > 0.5)) return false;
         if (h <= 1e-14) {
           *Zout = Z;
           return true;

@@ -45,10 +45,13 @@ constexpr double kPolarL0 = 3e-4;  // initial lower bound of the smallest singul
 // wrong, never convergence.  Returns false (and leaves *Zout undefined) when
 // max|T^H T - I| > start_tol, when the iteration stalls (a singular value
 // near 0) or when it does not reach max|Z^H Z - I| <= 1e-14 in kPolarMaxIter
-// steps.  Z0, Z1, P are n x n work buffers; dev holds kPolarMaxIter *
-// kPolarParts device doubles and hp as many pinned host doubles (the
-// read-backs of the deviation).  Synchronises the stream after the first
-// step, then after steps 9, 11, 13, ...
+// steps.  When Gershgorin's discs of Z^H Z at the first step leave out 0,
+// the square root of their lower end replaces kPolarL0 as the start bound.
+// Z0, Z1, P are n x n work buffers; dev holds kPolarMaxIter * kPolarParts
+// device doubles and hp as many pinned host doubles (the read-backs of the
+// deviation and of the bounds).  Synchronises the stream after the first
+// step, then every second step from two steps before the bound reaches 1
+// (from step 9 with the fixed bound).
 template <class T>
 bool polar_unitary(hipStream_t st, const T* Tin, int n, T* Z0, T* Z1, T* P, double* dev,
                    double* hp, double start_tol, T** Zout);
