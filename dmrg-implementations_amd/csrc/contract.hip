@@ -32,22 +32,29 @@ void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, cons
   const size_t need = (size_t)D * dd * cL * cR;
   DMRG_REQUIRE(need <= sc.cap, "heff2: scratch too small");
   const T one(1), zero(0);
-  // X[(w + D a'), (ss + dd b)] = sum_a L[a, w, a'] theta[a, ss, b]
-  if constexpr (std::is_same<T, float>::value) {
-    if (sc.split_bf16) {
-      gemm_f32_split_bf16(st, OP_T, OP_N, D * cL, dd * cR, cL, Lenv, cL, theta, cL, sc.X, D * cL);
+  // X[(w + D a'), (ss + dd b)] = sum_a L[a, w, a'] theta[a, ss, b], then
+  // Y[a' + cL ss' + cL dd b + cL dd cR n] = sum_{w,ss} X * WW[(w,ss),(ss',n)]:
+  // one fused launch where it applies (d = 2, real, not split-BF16)
+  bool fused = false;
+  if constexpr (!is_cplx<T>::value) {
+    if (!sc.split_bf16) fused = heff2_x_mpo<T>(st, cL, cR, d, D, Lenv, theta, WW, sc.Y);
+  }
+  if (!fused) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (sc.split_bf16) {
+        gemm_f32_split_bf16(st, OP_T, OP_N, D * cL, dd * cR, cL, Lenv, cL, theta, cL, sc.X, D * cL);
+      } else {
+        gemm<T>(st, OP_T, OP_N, D * cL, dd * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
+                pickT<T>(D * cL, dd * cR, cL));
+      }
     } else {
       gemm<T>(st, OP_T, OP_N, D * cL, dd * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
               pickT<T>(D * cL, dd * cR, cL));
     }
-  } else {
-    gemm<T>(st, OP_T, OP_N, D * cL, dd * cR, cL, one, Lenv, cL, theta, cL, zero, sc.X, D * cL,
-            pickT<T>(D * cL, dd * cR, cL));
+    View2 vx{{cL, (long long)D, (long long)D * cL * dd}, {D, 1, (long long)D * cL}};
+    View2 vy{{cL, 1, (long long)cL * dd}, {dd, (long long)cL, (long long)cL * dd * cR}};
+    mpo_step<T>(st, cL * cR, dd * D, D * dd, sc.X, vx, WW, sc.Y, vy);
   }
-  // Y[a' + cL ss' + cL dd b + cL dd cR n] = sum_{w,ss} X * WW[(w,ss),(ss',n)]
-  View2 vx{{cL, (long long)D, (long long)D * cL * dd}, {D, 1, (long long)D * cL}};
-  View2 vy{{cL, 1, (long long)cL * dd}, {dd, (long long)cL, (long long)cL * dd * cR}};
-  mpo_step<T>(st, cL * cR, dd * D, D * dd, sc.X, vx, WW, sc.Y, vy);
   // out[(a' + cL ss'), b'] = sum_{(b, n)} Y * R[(b + cR n), b']
   bool done = false;
   if constexpr (std::is_same<T, float>::value) {

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(int M, int N, int K, T alpha,
   B += blockIdx.z * sB;
   C += blockIdx.z * sC;
 
-  static_assert(PF == 1 || PF == 2, "prefetch depth");
+  static_assert(PF >= 1 && PF <= 4, "prefetch depth");
   T ra[PF][NA], rb[PF][NB];
 
   auto idxA = [&](int r, int& m, int& k) {
@@ -209,33 +209,26 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(int M, int N, int K, T alpha,
         }
     }
   };
+  // PF register stages: tile j travels in stage j % PF, then LDS buffer
+  // j & 1.  While tile kt is multiplied, tiles kt+1 .. kt+PF are in flight
+  // (kt+PF issued into the stage that tile kt just left); tile kt+1 is then
+  // staged into the other LDS buffer.  The loop is unrolled by PF so that
+  // stage indices are compile-time constants.
   load(0, 0);
   store(0, 0);
-  if constexpr (PF == 1) {
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-      const int buf = kt & 1;
-      if (kt + 1 < nk) load(0, (kt + 1) * BK);
+#pragma unroll
+  for (int s = 1; s < PF; ++s)
+    if (s < nk) load(s, s * BK);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int tile = kt + u;
+      if (tile >= nk) break;
+      const int buf = (PF % 2 == 0) ? (u & 1) : (tile & 1);  // kt is even when PF is
+      if (tile + PF < nk) load(u, (tile + PF) * BK);
       compute(buf);
-      if (kt + 1 < nk) store(0, buf ^ 1);
-      __syncthreads();
-    }
-  } else {
-    // Two register stages: the loads of tile kt+2 are in flight while tile
-    // kt is multiplied and tile kt+1 (loaded one step earlier) is staged.
-    // Stage and LDS buffer of tile kt are both kt & 1; the loop is unrolled
-    // by two so that stage indices are compile-time constants.
-    if (nk > 1) load(1, BK);
-    __syncthreads();
-    for (int kt = 0; kt < nk; kt += 2) {
-      if (kt + 2 < nk) load(0, (kt + 2) * BK);
-      compute(0);
-      if (kt + 1 < nk) store(1, 1);
-      __syncthreads();
-      if (kt + 1 >= nk) break;
-      if (kt + 3 < nk) load(1, (kt + 3) * BK);
-      compute(1);
-      if (kt + 2 < nk) store(0, 0);
+      if (tile + 1 < nk) store((u + 1) % PF, buf ^ 1);
       __syncthreads();
     }
   }
@@ -768,6 +761,141 @@ bool gemm_nn_splitk(hipStream_t st, int M, int N, int K, const T* A, int lda, co
   return true;
 }
 
+namespace {
+// First step of the two-site H_eff with the MPO-bond step applied in the
+// epilogue (d = 2, real scalars).  The workgroup computes the tile of
+// X = L^T theta with rows (w + D a') for 16 consecutive a' (all D values of w)
+// and columns (ss + 4 b) for 16 consecutive b (all four ss): wave j owns the
+// columns 16 j .. 16 j + 15 and all D row blocks, one 16x16 MFMA block each
+// per 4-deep k step, with the k-loop of k_gemm (two register stages, two LDS
+// buffers).  The finished tile goes to LDS (over the operand buffers, which
+// the k-loop no longer reads), and thread (a', b) contracts its 4D values
+// with WW into the 4D outputs Y[a' + cL ss' + 4 cL b + 4 cL cR n], summing
+// over c = w + D ss in the order of k_rows_small.  X never reaches HBM.
+template <class T, int D, int BK>
+__global__ __launch_bounds__(256) void k_heff2_x_mpo(int cL, int cR, const T* __restrict__ Lenv,
+                                                     const T* __restrict__ theta,
+                                                     const T* __restrict__ WW,
+                                                     T* __restrict__ Y) {
+  using V4 = typename GT<T>::V4;
+  constexpr int DD = 4, BM = 16 * D, BN = 16 * DD, K2 = D * DD;
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  static_assert(NA * 256 == BM * BK && NB * 256 == BN * BK, "tile");
+  constexpr int OPS = 2 * BK * (BM + 1) + 2 * BK * (BN + 1), XS = BM * (BN + 1);
+  __shared__ T smem[OPS > XS ? OPS : XS];
+  __shared__ T Ws[K2 * K2];
+  T(*As)[BK][BM + 1] = reinterpret_cast<T(*)[BK][BM + 1]>(smem);
+  T(*Bs)[BK][BN + 1] = reinterpret_cast<T(*)[BK][BN + 1]>(smem + 2 * BK * (BM + 1));
+  T(*Xs)[BN + 1] = reinterpret_cast<T(*)[BN + 1]>(smem);
+  const int t = threadIdx.x, lane = t & 63, wn = t >> 6;
+  const int M = D * cL, N = DD * cR, K = cL;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  for (int e = t; e < K2 * K2; e += 256) Ws[e] = WW[e];
+  // A(m, k) = Lenv[k + m cL], B(k, n) = theta[k + n cL]: both k-fast
+  T ra[2][NA], rb[2][NB];
+  const bool inner = m0 + BM <= M && n0 + BN <= N;
+  auto load = [&](int s, int k0) {
+    const bool full = inner && k0 + BK <= K;
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      const int k = t % BK, m = t / BK + r * (256 / BK);
+      const int gm = m0 + m, gk = k0 + k;
+      ra[s][r] = (full || (gm < M && gk < K)) ? Lenv[gk + (size_t)gm * cL] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      const int k = t % BK, n = t / BK + r * (256 / BK);
+      const int gn = n0 + n, gk = k0 + k;
+      rb[s][r] = (full || (gn < N && gk < K)) ? theta[gk + (size_t)gn * cL] : T(0);
+    }
+  };
+  auto store = [&](int s, int buf) {
+#pragma unroll
+    for (int r = 0; r < NA; ++r) As[buf][t % BK][t / BK + r * (256 / BK)] = ra[s][r];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) Bs[buf][t % BK][t / BK + r * (256 / BK)] = rb[s][r];
+  };
+  V4 acc[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) acc[i] = V4{0, 0, 0, 0};
+  const int li = lane & 15, lk = lane >> 4;
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) af[i] = As[buf][kk + lk][i * 16 + li];
+      const T bf = Bs[buf][kk + lk][wn * 16 + li];
+#pragma unroll
+      for (int i = 0; i < D; ++i) acc[i] = mfma(bf, af[i], acc[i]);
+    }
+  };
+  const int nk = (K + BK - 1) / BK;
+  load(0, 0);
+  store(0, 0);
+  if (nk > 1) load(1, BK);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt += 2) {
+    if (kt + 2 < nk) load(0, (kt + 2) * BK);
+    compute(0);
+    if (kt + 1 < nk) store(1, 1);
+    __syncthreads();
+    if (kt + 1 >= nk) break;
+    if (kt + 3 < nk) load(1, (kt + 3) * BK);
+    compute(1);
+    if (kt + 2 < nk) store(0, 0);
+    __syncthreads();
+  }
+  // every exit of the loop passed a barrier: the operand buffers are free
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      Xs[i * 16 + li][wn * 16 + (GT<T>::G1 ? lk + 4 * r : 4 * lk + r)] = acc[i][r];
+  __syncthreads();
+  const int al = t & 15, bl = t >> 4;
+  const int a = blockIdx.x * 16 + al, b = blockIdx.y * 16 + bl;
+  if (a >= cL || b >= cR) return;
+  T x[K2];
+#pragma unroll
+  for (int w = 0; w < D; ++w)
+#pragma unroll
+    for (int ss = 0; ss < DD; ++ss) x[w + D * ss] = Xs[w + D * al][ss + DD * bl];
+  T* const yb = Y + a + (size_t)cL * DD * b;
+  const size_t sn = (size_t)cL * DD * cR;
+#pragma unroll
+  for (int n = 0; n < D; ++n)
+#pragma unroll
+    for (int sp = 0; sp < DD; ++sp) {
+      const int c2 = sp + DD * n;
+      T y = T(0);
+#pragma unroll
+      for (int c = 0; c < K2; ++c) y = y + x[c] * Ws[c + K2 * c2];
+      yb[(size_t)cL * sp + sn * n] = y;
+    }
+}
+}  // namespace
+
+template <class T>
+bool heff2_x_mpo(hipStream_t st, int cL, int cR, int d, int D, const T* Lenv, const T* theta,
+                 const T* WW, T* Y) {
+  if constexpr (is_cplx<T>::value) {
+    return false;
+  } else {
+    if (d != 2 || D < 2 || D > 8 || cL < 1 || cR < 1) return false;
+    const dim3 grid(cdiv(cL, 16), cdiv(cR, 16));
+    switch (D) {
+#define DMRG_XM(D_) \
+  case D_: hipLaunchKernelGGL((k_heff2_x_mpo<T, D_, 16>), grid, dim3(256), 0, st, cL, cR, Lenv, theta, WW, Y); break
+      DMRG_XM(2); DMRG_XM(3); DMRG_XM(4); DMRG_XM(5); DMRG_XM(6); DMRG_XM(7); DMRG_XM(8);
+#undef DMRG_XM
+      default: return false;
+    }
+    DMRG_LAUNCH_CHECK();
+    return true;
+  }
+}
+
 template <class T>
 bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, const T* B, T* C,
                      View2 vc) {
@@ -858,7 +986,8 @@ void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2
   template bool gemm_rows_small<T>(hipStream_t, int, int, int, const T*, View2, const T*, T*,  \
                                    View2);                                                      \
   template bool gemm_nn_splitk<T>(hipStream_t, int, int, int, const T*, int, const T*, int, T*, \
-                                  int, T*, size_t);
+                                  int, T*, size_t);                                             \
+  template bool heff2_x_mpo<T>(hipStream_t, int, int, int, int, const T*, const T*, const T*, T*);
 DMRG_INST(double)
 DMRG_INST(cdouble)
 DMRG_INST(float)

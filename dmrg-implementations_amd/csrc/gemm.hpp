@@ -72,6 +72,17 @@ template <class T>
 bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, const T* B, T* C,
                      View2 vc);
 
+// The two-site H_eff's first two steps in one launch (d = 2, real scalars,
+// 2 <= D <= 8): Y[a' + cL ss' + cL d^2 b + cL d^2 cR n] =
+//   sum_{w, ss} X[(w + D a'), (ss + d^2 b)] WW[(w + D ss) + D d^2 (ss' + d^2 n)]
+// with X = L^T theta (L: cL x (D cL), theta: cL x (d^2 cR), both ld cL)
+// formed tile by tile on the MFMA and contracted with WW in LDS, so X never
+// reaches HBM; the same sums in the same order as the GEMM followed by
+// gemm_rows_small.  Returns false (and does nothing) outside that range.
+template <class T>
+bool heff2_x_mpo(hipStream_t st, int cL, int cR, int d, int D, const T* Lenv, const T* theta,
+                 const T* WW, T* Y);
+
 // FP32 C (M x N, ldc) = op(A) op(B) with split-BF16 products: each operand as
 // three bf16 terms, the six leading cross products on the bf16 MFMA with FP32
 // accumulation (FP32-class accuracy).  op: OP_N or OP_T (conjugation is a

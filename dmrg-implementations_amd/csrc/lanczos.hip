@@ -313,47 +313,38 @@ __device__ inline int sturm_count(const double* a, const double* b2, int k, doub
 }
 
 // Lowest eigenvalue by 64-way multisection (one wave).
-__device__ double tri_lowest(const double* a, const double* b, int k, double* b2, int* cnt) {
+__device__ double tri_lowest(const double* a, const double* b, int k, double* b2) {
+  // One wave: every lane holds the same bracket [lo, hi] (the same arithmetic
+  // on the same data), evaluates the Sturm count at one of 64 interior points
+  // per round, and a ballot gives the first point with a count >= 1.
   const int t = threadIdx.x;
-  __shared__ double lohi[2];
-  __shared__ double pm;
-  if (t == 0) {
-    double lo = a[0], hi = a[0], mb = 0.0;
-    for (int i = 0; i < k; ++i) {
-      double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0);
-      lo = fmin(lo, a[i] - r);
-      hi = fmax(hi, a[i] + r);
-      if (i < k - 1) mb = fmax(mb, b[i] * b[i]);
-    }
-    double w = fmax(fabs(lo), fabs(hi));
+  double lo = a[0], hi = a[0], mb = 0.0;
+  for (int i = 0; i < k; ++i) {
+    double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+    if (i < k - 1) mb = fmax(mb, b[i] * b[i]);
+  }
+  {
+    const double w = fmax(fabs(lo), fabs(hi));
     lo -= 2.0 * DBL_EPSILON * w + DBL_MIN;
     hi += 2.0 * DBL_EPSILON * w + DBL_MIN;
-    lohi[0] = lo;
-    lohi[1] = hi;
-    pm = DBL_MIN * fmax(1.0, mb);
   }
-  for (int i = t; i < k - 1; i += blockDim.x) b2[i] = b[i] * b[i];
+  const double pm = DBL_MIN * fmax(1.0, mb);
+  for (int i = t; i < k - 1; i += 64) b2[i] = b[i] * b[i];
   __syncthreads();
   for (int round = 0; round < 24; ++round) {
-    const double lo = lohi[0], hi = lohi[1];
     const double w = fmax(fabs(lo), fabs(hi));
     if (hi - lo <= 2.0 * DBL_EPSILON * w + 4.0 * DBL_MIN) break;
     const double x = lo + (hi - lo) * (double)(t + 1) / 65.0;
-    cnt[t] = sturm_count(a, b2, k, x, pm) >= 1;
-    __syncthreads();
-    if (t == 0) {
-      int l = 0;
-      while (l < 64 && !cnt[l]) ++l;
-      double nlo = (l == 0) ? lo : lo + (hi - lo) * (double)l / 65.0;
-      double nhi = (l == 64) ? hi : lo + (hi - lo) * (double)(l + 1) / 65.0;
-      lohi[0] = nlo;
-      lohi[1] = nhi;
-    }
-    __syncthreads();
+    const unsigned long long m = __ballot(sturm_count(a, b2, k, x, pm) >= 1);
+    const int l = m ? __ffsll((long long)m) - 1 : 64;
+    const double nlo = (l == 0) ? lo : lo + (hi - lo) * (double)l / 65.0;
+    const double nhi = (l == 64) ? hi : lo + (hi - lo) * (double)(l + 1) / 65.0;
+    lo = nlo;
+    hi = nhi;
   }
-  double r = 0.5 * (lohi[0] + lohi[1]);
-  __syncthreads();
-  return r;
+  return 0.5 * (lo + hi);
 }
 
 // Eigenvector of T for eigenvalue lam by inverse iteration (thread 0).
@@ -425,7 +416,6 @@ __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* b
                                            double beta_tol, double eig_tol, double eig_rel,
                                            double* scal, int* status, double* y) {
   __shared__ double b2[128];
-  __shared__ int cnt[64];
   __shared__ int bidx;
   __shared__ double wd[128], wdu[128], wdl[128], wdu2[128], wx[128];
   __shared__ int wpiv[128];
@@ -441,7 +431,7 @@ __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* b
       if (t == 0) { status[0] = 1; status[1] = bidx + 1; }
       return;
     }
-    double lam = tri_lowest(alpha, beta, k, b2, cnt);
+    double lam = tri_lowest(alpha, beta, k, b2);
     if (t == 0) {
       scal[4] = lam;
       double prev = scal[2];
@@ -449,7 +439,7 @@ __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* b
       else { status[0] = 0; scal[2] = lam; }
     }
   } else {  // final
-    double lam = (k == 1) ? alpha[0] : tri_lowest(alpha, beta, k, b2, cnt);
+    double lam = (k == 1) ? alpha[0] : tri_lowest(alpha, beta, k, b2);
     if (t == 0) {
       scal[3] = lam;
       tri_vector(alpha, beta, k, lam, y, wd, wdu, wdl, wdu2, wx, wpiv);
