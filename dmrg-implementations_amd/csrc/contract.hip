@@ -73,6 +73,32 @@ void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, cons
             cL * dd, pickT<T>(cL * dd, cR, D * cR));
 }
 
+void heff2_apply_f32_io64(hipStream_t st, const float* Lenv, const float* Renv, const float* WW,
+                          const double* theta, double* out, int cL, int cR, int d, int D,
+                          ContractScratch<float>& sc, float* thf, float* outf, const double* sL,
+                          const double* sR) {
+  const int dd = d * d;
+  const size_t n = (size_t)cL * dd * cR;
+  const size_t need = (size_t)D * dd * cL * cR;
+  DMRG_REQUIRE(need <= sc.cap, "heff2: scratch too small");
+  // theta is converted once: every row tile of the first step reads it, and
+  // reading it as FP64 there doubles that L2 traffic (measured: the C4
+  // matvec 14 us slower than the separate conversion)
+  convert_f64_f32(st, theta, thf, n);
+  if (!sc.split_bf16 && heff2_x_mpo<float>(st, cL, cR, d, D, Lenv, thf, WW, sc.Y)) {
+    // X is free: it serves as the split-K workspace
+    if (gemm_nn_splitk_to_f64(st, cL * dd, cR, D * cR, sc.Y, cL * dd, Renv, cR * D, out, cL * dd,
+                              sc.X, sc.cap, sL, sR))
+      return;
+    gemm<float>(st, OP_N, OP_N, cL * dd, cR, D * cR, 1.f, sc.Y, cL * dd, Renv, cR * D, 0.f, outf,
+                cL * dd, pickT<float>(cL * dd, cR, D * cR));
+    convert_f32_f64_scaled(st, outf, out, n, sL, sR);
+    return;
+  }
+  heff2_apply<float>(st, Lenv, Renv, WW, thf, outf, cL, cR, d, D, sc);
+  convert_f32_f64_scaled(st, outf, out, n, sL, sR);
+}
+
 template <class T>
 void heff1_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* W, const T* theta, T* out,
                  int cL, int cR, int d, int D, ContractScratch<T>& sc) {

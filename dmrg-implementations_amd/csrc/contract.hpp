@@ -31,6 +31,17 @@ template <class T>
 void heff2_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* WW, const T* theta, T* out,
                  int cL, int cR, int d, int D, ContractScratch<T>& sc);
 
+// The FP32 two-site H_eff on FP64 vectors: out = sL sR H_eff(FP32 L, R, WW)
+// theta, with L and R scaled to unit max-abs by sL and sR (device scalars).
+// theta is rounded to FP32 into thf; where the fused first step and the
+// split-K out-step apply, the split-K sum writes the widened, rescaled result
+// directly, otherwise the FP32 result goes through outf (n elements each)
+// and a conversion.  Same values either way.
+void heff2_apply_f32_io64(hipStream_t st, const float* Lenv, const float* Renv, const float* WW,
+                          const double* theta, double* out, int cL, int cR, int d, int D,
+                          ContractScratch<float>& sc, float* thf, float* outf, const double* sL,
+                          const double* sR);
+
 // Single-site H_eff (dmrg_gpu_impl.h:538-705); W is the raw MPO tensor.
 template <class T>
 void heff1_apply(hipStream_t st, const T* Lenv, const T* Renv, const T* W, const T* theta, T* out,

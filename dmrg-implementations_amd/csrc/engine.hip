@@ -629,16 +629,12 @@ void Engine<T>::matvec2(int site, const T* in, T* out, bool fp64) {
     heff2_apply<T>(s_main, Lenv[site], Renv[site + 2], WWd[site], in, out, cL, cR, d, D, sc_main);
   } else {
     if constexpr (!is_cplx<T>::value) {
-      const size_t n = (size_t)cL * d * d * cR;
-      convert_f64_f32(s_main, in, thf, n);
-      if (env32) {
-        heff2_apply<float>(s_main, Lfe[site], Rfe[site + 2], WWf[site], thf, outf, cL, cR, d, D,
-                           sc_f);
-        convert_f32_f64_scaled(s_main, outf, out, n, envS + site, envS + (L + 1) + site + 2);
-      } else {
-        heff2_apply<float>(s_main, Lf, Rf, WWf[site], thf, outf, cL, cR, d, D, sc_f);
-        convert_f32_f64_scaled(s_main, outf, out, n, envScale, envScale + 1);
-      }
+      if (env32)
+        heff2_apply_f32_io64(s_main, Lfe[site], Rfe[site + 2], WWf[site], in, out, cL, cR, d, D,
+                             sc_f, thf, outf, envS + site, envS + (L + 1) + site + 2);
+      else
+        heff2_apply_f32_io64(s_main, Lf, Rf, WWf[site], in, out, cL, cR, d, D, sc_f, thf, outf,
+                             envScale, envScale + 1);
     }
   }
 }

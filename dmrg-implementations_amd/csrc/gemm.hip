@@ -722,22 +722,32 @@ void gemm_f32_split_bf16(hipStream_t st, int opA, int opB, int M, int N, int K, 
 namespace {
 // C[i] = sum_{z<S} W[i + z*stride] for i < M*N over an M x N block (ld ldc);
 // fixed summation order (deterministic).
-template <class T>
+// FP32 partials into an FP64 result (TO = double, T = float): the sum is
+// formed in FP32 as for an FP32 result, then widened and multiplied by
+// s1 * s2 (each taken as 1 when not positive), the factor that undoes the
+// unit-max scaling of the FP32 operands.
+template <class T, class TO>
 __global__ void k_splitk_sum(const T* __restrict__ W, int S, long long stride, int M, int N,
-                             T* __restrict__ C, int ldc) {
+                             TO* __restrict__ C, int ldc, const double* __restrict__ s1,
+                             const double* __restrict__ s2) {
+  double f = 1.0;
+  if constexpr (!std::is_same<T, TO>::value) {
+    const double a = *s1, b = *s2;
+    f = (a > 0.0 ? a : 1.0) * (b > 0.0 ? b : 1.0);
+  }
   const long long tot = (long long)M * N;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
        e += (long long)gridDim.x * blockDim.x) {
     T v = W[e];
     for (int z = 1; z < S; ++z) v = v + W[e + z * stride];
-    C[(e % M) + (e / M) * (long long)ldc] = v;
+    if constexpr (std::is_same<T, TO>::value) C[(e % M) + (e / M) * (long long)ldc] = v;
+    else C[(e % M) + (e / M) * (long long)ldc] = (TO)v * f;
   }
 }
-}  // namespace
 
-template <class T>
-bool gemm_nn_splitk(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
-                    T* C, int ldc, T* ws, size_t ws_elems) {
+template <class T, class TO>
+bool splitk_impl(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
+                 TO* C, int ldc, T* ws, size_t ws_elems, const double* s1, const double* s2) {
   // S equal slices (S | K) in one batched launch, while the 64x64 grid
   // leaves CUs idle and each slice keeps K/S >= 512.  FP32 (BK = 32 tiles)
   // stops at two slices: measured 80 TF/s against 74 for four on the C4
@@ -755,10 +765,24 @@ bool gemm_nn_splitk(hipStream_t st, int M, int N, int K, const T* A, int lda, co
   gemm_strided<T>(st, OP_N, OP_N, M, N, Kc, T(1), A, lda, (long long)Kc * lda, B, ldb,
                   (long long)Kc, T(0), ws, M, (long long)M * N, S, GemmTile{64, 64, 2});
   const long long tot = (long long)M * N;
-  hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)std::min<long long>(cdiv(tot, 256), 4096)),
-                     dim3(256), 0, st, (const T*)ws, S, (long long)M * N, M, N, C, ldc);
+  hipLaunchKernelGGL((k_splitk_sum<T, TO>),
+                     dim3((unsigned)std::min<long long>(cdiv(tot, 256), 4096)), dim3(256), 0, st,
+                     (const T*)ws, S, (long long)M * N, M, N, C, ldc, s1, s2);
   DMRG_LAUNCH_CHECK();
   return true;
+}
+}  // namespace
+
+template <class T>
+bool gemm_nn_splitk(hipStream_t st, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
+                    T* C, int ldc, T* ws, size_t ws_elems) {
+  return splitk_impl<T, T>(st, M, N, K, A, lda, B, ldb, C, ldc, ws, ws_elems, nullptr, nullptr);
+}
+
+bool gemm_nn_splitk_to_f64(hipStream_t st, int M, int N, int K, const float* A, int lda,
+                           const float* B, int ldb, double* C, int ldc, float* ws,
+                           size_t ws_elems, const double* s1, const double* s2) {
+  return splitk_impl<float, double>(st, M, N, K, A, lda, B, ldb, C, ldc, ws, ws_elems, s1, s2);
 }
 
 namespace {
@@ -772,9 +796,11 @@ namespace {
 // the k-loop no longer reads), and thread (a', b) contracts its 4D values
 // with WW into the 4D outputs Y[a' + cL ss' + 4 cL b + 4 cL cR n], summing
 // over c = w + D ss in the order of k_rows_small.  X never reaches HBM.
-template <class T, int D, int BK>
+// theta may be FP64 for an FP32 product (TB = double, T = float): it is
+// rounded to FP32 on load, as a separate conversion would round it.
+template <class T, class TB, int D, int BK>
 __global__ __launch_bounds__(256) void k_heff2_x_mpo(int cL, int cR, const T* __restrict__ Lenv,
-                                                     const T* __restrict__ theta,
+                                                     const TB* __restrict__ theta,
                                                      const T* __restrict__ WW,
                                                      T* __restrict__ Y) {
   using V4 = typename GT<T>::V4;
@@ -806,7 +832,7 @@ __global__ __launch_bounds__(256) void k_heff2_x_mpo(int cL, int cR, const T* __
     for (int r = 0; r < NB; ++r) {
       const int k = t % BK, n = t / BK + r * (256 / BK);
       const int gn = n0 + n, gk = k0 + k;
-      rb[s][r] = (full || (gn < N && gk < K)) ? theta[gk + (size_t)gn * cL] : T(0);
+      rb[s][r] = (full || (gn < N && gk < K)) ? (T)theta[gk + (size_t)gn * cL] : T(0);
     }
   };
   auto store = [&](int s, int buf) {
@@ -876,17 +902,17 @@ __global__ __launch_bounds__(256) void k_heff2_x_mpo(int cL, int cR, const T* __
 }
 }  // namespace
 
-template <class T>
-bool heff2_x_mpo(hipStream_t st, int cL, int cR, int d, int D, const T* Lenv, const T* theta,
+template <class T, class TB>
+bool heff2_x_mpo(hipStream_t st, int cL, int cR, int d, int D, const T* Lenv, const TB* theta,
                  const T* WW, T* Y) {
-  if constexpr (is_cplx<T>::value) {
+  if constexpr (is_cplx<T>::value || is_cplx<TB>::value) {
     return false;
   } else {
     if (d != 2 || D < 2 || D > 8 || cL < 1 || cR < 1) return false;
     const dim3 grid(cdiv(cL, 16), cdiv(cR, 16));
     switch (D) {
 #define DMRG_XM(D_) \
-  case D_: hipLaunchKernelGGL((k_heff2_x_mpo<T, D_, 16>), grid, dim3(256), 0, st, cL, cR, Lenv, theta, WW, Y); break
+  case D_: hipLaunchKernelGGL((k_heff2_x_mpo<T, TB, D_, 16>), grid, dim3(256), 0, st, cL, cR, Lenv, theta, WW, Y); break
       DMRG_XM(2); DMRG_XM(3); DMRG_XM(4); DMRG_XM(5); DMRG_XM(6); DMRG_XM(7); DMRG_XM(8);
 #undef DMRG_XM
       default: return false;
@@ -987,10 +1013,13 @@ void gemm_tensor(hipStream_t st, int M, int N, int K, T alpha, const T* A, View2
                                    View2);                                                      \
   template bool gemm_nn_splitk<T>(hipStream_t, int, int, int, const T*, int, const T*, int, T*, \
                                   int, T*, size_t);                                             \
-  template bool heff2_x_mpo<T>(hipStream_t, int, int, int, int, const T*, const T*, const T*, T*);
+  template bool heff2_x_mpo<T, T>(hipStream_t, int, int, int, int, const T*, const T*, const T*, \
+                                  T*);
 DMRG_INST(double)
 DMRG_INST(cdouble)
 DMRG_INST(float)
 #undef DMRG_INST
+template bool heff2_x_mpo<float, double>(hipStream_t, int, int, int, int, const float*,
+                                         const double*, const float*, float*);
 
 }  // namespace dmrg

@@ -79,9 +79,17 @@ bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, 
 // formed tile by tile on the MFMA and contracted with WW in LDS, so X never
 // reaches HBM; the same sums in the same order as the GEMM followed by
 // gemm_rows_small.  Returns false (and does nothing) outside that range.
-template <class T>
-bool heff2_x_mpo(hipStream_t st, int cL, int cR, int d, int D, const T* Lenv, const T* theta,
+// TB = double with T = float: an FP64 theta rounded to FP32 on load.
+template <class T, class TB = T>
+bool heff2_x_mpo(hipStream_t st, int cL, int cR, int d, int D, const T* Lenv, const TB* theta,
                  const T* WW, T* Y);
+
+// gemm_nn_splitk with FP32 operands and an FP64 result C = s1 s2 (A B)
+// (s1, s2 device scalars, each taken as 1 when not positive); the partial
+// products and their sum stay FP32.
+bool gemm_nn_splitk_to_f64(hipStream_t st, int M, int N, int K, const float* A, int lda,
+                           const float* B, int ldb, double* C, int ldc, float* ws,
+                           size_t ws_elems, const double* s1, const double* s2);
 
 // FP32 C (M x N, ldc) = op(A) op(B) with split-BF16 products: each operand as
 // three bf16 terms, the six leading cross products on the bf16 MFMA with FP32
