@@ -416,10 +416,18 @@ __global__ __launch_bounds__(64) void k_tri(const double* alpha, const double* b
                                            double beta_tol, double eig_tol, double eig_rel,
                                            double* scal, int* status, double* y) {
   __shared__ double b2[128];
+  __shared__ double as[128], bs[128];  // the tridiagonal, read once from HBM
   __shared__ int bidx;
   __shared__ double wd[128], wdu[128], wdl[128], wdu2[128], wx[128];
   __shared__ int wpiv[128];
   const int t = threadIdx.x;
+  for (int i = t; i < k; i += 64) {
+    as[i] = alpha[i];
+    bs[i] = beta[i];
+  }
+  __syncthreads();
+  alpha = as;
+  beta = bs;
   if (mode == 0) {  // check
     if (t == 0) {
       bidx = -1;
