@@ -263,7 +263,7 @@ class DMRG:
         _check(lib().dmrg_run(self._h, n_sweeps, 1 if quiet else 0, C.byref(e)))
         return e.value
 
-    def run_pdmrg(self, segments=2, outer=10, local=2, warmup=1, polish=1, quiet=True):
+    def run_pdmrg(self, segments=2, outer=10, local=2, warmup=1, polish=2, quiet=True):
         """Segment-parallel two-site sweep (dmrg_run_pdmrg)."""
         e = C.c_double(0)
         _check(lib().dmrg_run_pdmrg(self._h, segments, outer, local, warmup, polish,

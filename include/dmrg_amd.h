@@ -205,8 +205,10 @@ int dmrg_run_batch(dmrg_handle* hs, int n, int n_sweeps, int threads, double* en
  * and then the reverse, each pass followed by the boundary merges
  * theta = psi_L diag(V) psi_R with V = 1/max(S, 1e-12) of the boundary's last
  * split; stops when the energy changes by less than tol.  Then `polish`
- * serial full sweeps.  Two-site FP64 / complex128 only (BAD_ARG otherwise).
- * The energy is the last merge's, or the last polish sweep's. */
+ * serial full sweeps (two or more at large chi: the merges' energy is not
+ * monotone there, and the polish sweeps restore the serial result).  Two-site
+ * FP64 / complex128 only (BAD_ARG otherwise).  The energy is the last
+ * merge's, or the last polish sweep's. */
 int dmrg_run_pdmrg(dmrg_handle h, int segments, int outer, int local, int warmup, int polish,
                    int quiet, double* energy);
 
