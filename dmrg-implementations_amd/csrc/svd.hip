@@ -955,19 +955,16 @@ __global__ void k_add_diag(T* __restrict__ X, int n, double b) {
 }
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
-// Modified Gram-Schmidt, twice, on the b <= MGS_B columns of one block of
-// Y (len rows, ld ldy), by one workgroup: for each column j, the norm and
-// the inner products with the later columns come from one pass over the
-// rows, the later columns are projected, and column j is normalised.  A
-// column whose norm falls below `dep` times its norm before the
-// orthogonalisation (nrm0, device) is numerically dependent: it is set to
-// zero (the caller's sketch then simply carries fewer directions).
 constexpr int MGS_B = 32;
+// Modified Gram-Schmidt, twice, on the block (the fallback of k_block_orth
+// for an ill-conditioned block): for each column j, the norm and the inner
+// products with the later columns come from one pass over the rows, the
+// later columns are projected, and column j is normalised; a column below
+// `dep` times its norm before the orthogonalisation becomes zero.
 template <class T>
-__global__ __launch_bounds__(256) void k_mgs_block(T* __restrict__ Y, int len, int ldy, int b,
-                                                   const double* __restrict__ nrm0, double dep) {
-  __shared__ T red[MGS_B][8];  // per-wave partial sums (4 waves) of <y_j, y_c>
-  __shared__ double nred[8];
+__device__ void mgs_block(T* __restrict__ Y, int len, int ldy, int b,
+                          const double* __restrict__ nrm0, double dep, T (*red)[8],
+                          double* nred) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int pass = 0; pass < 2; ++pass) {
     for (int j = 0; j < b; ++j) {
@@ -1016,6 +1013,128 @@ __global__ __launch_bounds__(256) void k_mgs_block(T* __restrict__ Y, int len, i
       for (int i = t; i < len; i += 256) yj[i] = yj[i] * inv;
       __syncthreads();
     }
+  }
+}
+
+// Orthonormalisation of the b <= MGS_B columns of one block of Y (len rows,
+// ld ldy) by one workgroup, as Cholesky QR twice: the Gram matrix
+// G = Y_b^H Y_b is accumulated in LDS over chunks of rows, factored
+// G = R^H R in LDS, and every row y of the block is replaced by y R^-1
+// (forward substitution, one thread per row).  Column j is numerically
+// dependent when its squared norm after the earlier columns' projection,
+// G_jj - sum_i |R_ij|^2, is not above (dep * ref_j)^2 and 1e-14 G_jj (the
+// Gram matrix's own rounding), ref_j being its norm before any
+// orthogonalisation (nrm0, device) in the first pass and 1 in the second; a
+// dependent column is set to zero (the caller's sketch then simply carries
+// fewer directions).  One pass over the block's rows per Gram matrix instead
+// of MGS's b passes: the block of a sketch, projected against the earlier
+// blocks, is well conditioned, so the squared condition number that Cholesky
+// QR costs is harmless, and the second pass restores orthogonality to
+// rounding.  A block whose first factorisation finds a column with less than
+// 1e-4 of its norm left after the earlier columns' projection (condition
+// number above ~1e4, where Cholesky QR would lose directions that MGS keeps)
+// is left untouched by it and orthonormalised by MGS instead.
+template <class T>
+__global__ __launch_bounds__(256) void k_block_orth(T* __restrict__ Y, int len, int ldy, int b,
+                                                    const double* __restrict__ nrm0,
+                                                    double dep) {
+  constexpr int CH = 32;                 // rows per chunk of the Gram accumulation
+  __shared__ T Ys[CH][MGS_B + 1];        // chunk of the block
+  __shared__ T G[MGS_B][MGS_B + 1];      // Gram matrix, then R (upper triangle)
+  __shared__ double Gd0[MGS_B];          // G_jj before the factorisation
+  __shared__ int kept[MGS_B];
+  __shared__ int ill;
+  const int t = threadIdx.x;
+  // (p, q) entries of the upper triangle owned by thread t: e = t, t + 256, ...
+  constexpr int NE = MGS_B * (MGS_B + 1) / 2, NU = (NE + 255) / 256;
+  int pu[NU], qu[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    int p = 0, rem = t + 256 * u;  // row-major upper triangle: e -> (p, q), p <= q
+    while (p < MGS_B && rem >= MGS_B - p) { rem -= MGS_B - p; ++p; }
+    pu[u] = p;
+    qu[u] = p + rem;
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+    T acc[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) acc[u] = T(0);
+    for (int r0 = 0; r0 < len; r0 += CH) {
+      for (int e = t; e < CH * MGS_B; e += 256) {
+        const int i = e % CH, c = e / CH;
+        Ys[i][c] = (r0 + i < len && c < b) ? Y[r0 + i + (size_t)c * ldy] : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        if (t + 256 * u < NE) {
+          const int p = pu[u], q = qu[u];
+          T sacc = acc[u];
+          for (int i = 0; i < CH; ++i) sacc += conj(Ys[i][p]) * Ys[i][q];
+          acc[u] = sacc;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      if (t + 256 * u < NE) {
+        const int p = pu[u], q = qu[u];
+        G[p][q] = acc[u];
+        if (p == q) Gd0[p] = real(acc[u]);
+      }
+    }
+    __syncthreads();
+    // right-looking Cholesky with dependent columns dropped
+    if (t == 0) ill = 0;
+    for (int j = 0; j < b; ++j) {
+      if (t == 0) {
+        const double d = real(G[j][j]);
+        const double ref = pass == 0 ? nrm0[j] : 1.0;
+        const double tiny = fmax(dep * dep * ref * ref, 1e-14 * Gd0[j]);
+        kept[j] = (d == d) && d > tiny;
+        if (pass == 0 && !(d > 1e-8 * Gd0[j]) && Gd0[j] > 0.0) ill = 1;
+        G[j][j] = T(kept[j] ? sqrt(d) : 0.0);
+      }
+      __syncthreads();
+      const bool kj = kept[j];
+      const double inv = kj ? 1.0 / real(G[j][j]) : 0.0;
+      for (int q = j + 1 + t; q < b; q += 256) G[j][q] = kj ? G[j][q] * inv : T(0);
+      __syncthreads();
+      if (kj) {
+        const int m = b - j - 1;  // trailing (p, q), j < p <= q < b
+        for (int e = t; e < m * m; e += 256) {
+          const int p = j + 1 + e % m, q = j + 1 + e / m;
+          if (p <= q) G[p][q] -= conj(G[j][p]) * G[j][q];
+        }
+      }
+      __syncthreads();
+    }
+    if (ill) {  // (set by thread 0 before the last barrier of the loop)
+      __shared__ T red[MGS_B][8];
+      __shared__ double nred[8];
+      mgs_block<T>(Y, len, ldy, b, nrm0, dep, red, nred);
+      return;
+    }
+    // rows: y <- y R^-1 (dependent columns become zero)
+    for (int i = t; i < len; i += 256) {
+      T x[MGS_B];
+#pragma unroll
+      for (int c = 0; c < MGS_B; ++c) x[c] = c < b ? Y[i + (size_t)c * ldy] : T(0);
+#pragma unroll
+      for (int c = 0; c < MGS_B; ++c) {
+        if (c < b) {
+          T v = x[c];
+#pragma unroll
+          for (int k = 0; k < c; ++k) v -= x[k] * G[k][c];
+          x[c] = kept[c] ? v * (1.0 / real(G[c][c])) : T(0);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < MGS_B; ++c)
+        if (c < b) Y[i + (size_t)c * ldy] = x[c];
+    }
+    __syncthreads();
   }
 }
 
@@ -1486,7 +1605,8 @@ int heig(hipStream_t st, SvdWork<T>& w, const T* H, int N, int ldh, double* eval
 // Orthonormal basis of the columns of Y (len x r, ld ldy), in place, by
 // block classical Gram-Schmidt with reorthogonalisation (each block of
 // MGS_B columns projected twice against the finished ones by GEMMs, then
-// MGS twice inside the block).  Numerically dependent columns become zero.
+// Cholesky QR twice, or MGS twice for an ill-conditioned block, inside the
+// block).  Numerically dependent columns become zero.
 // C (>= r * MGS_B scalars) and nrm (>= r doubles) are device work space.
 template <class T>
 void orth_columns(hipStream_t st, T* Y, int len, int r, int ldy, T* C, double* nrm) {
@@ -1503,7 +1623,7 @@ void orth_columns(hipStream_t st, T* Y, int len, int r, int ldy, T* C, double* n
       gemm<T>(st, OP_N, OP_N, len, b, j0, T(-1), Y, ldy, C, j0, T(1), Yj, ldy,
               gemm_pick_tile(len, b, j0, cx));
     }
-    hipLaunchKernelGGL(k_mgs_block<T>, dim3(1), dim3(256), 0, st, Yj, len, ldy, b,
+    hipLaunchKernelGGL(k_block_orth<T>, dim3(1), dim3(256), 0, st, Yj, len, ldy, b,
                        (const double*)(nrm + j0), 1e-13);
     DMRG_LAUNCH_CHECK();
   }
