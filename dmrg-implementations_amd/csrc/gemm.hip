@@ -361,13 +361,14 @@ void dispatch(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K
 
 // ---------------------------------------------------------------------------
 // Tile autotuner.  When enabled (gemm_autotune), the first non-batched call
-// of each (scalar type, ops, M, N, K, Hermitian) shape times every tile
-// (32x32 and 64x64, with four or eight waves) on
-// the caller's stream, with the device otherwise idle (one warm-up each, then
+// of each (scalar type, ops, M, N, K, Hermitian) shape whose caller tile has
+// one wave per output sub-tile times the 32x32 and 64x64 tiles on the
+// caller's stream, with the device otherwise idle (one warm-up each, then
 // four interleaved rounds of one launch each into a scratch output, keeping
 // each tile's fastest), and keeps the fastest for the rest of the process --
-// the caller's tile unless another is at least 3 % faster; later calls of
-// that shape use it.  A call made while its stream is being captured
+// the caller's tile unless the other is at least 3 % faster; later calls of
+// that shape use it.  Both tiles sum over k in the same order, so tuning
+// never changes a result: runs are bit-reproducible across processes.  A call made while its stream is being captured
 // into a graph uses the caller's tile and records nothing.
 // ---------------------------------------------------------------------------
 struct TuneKey {
@@ -427,8 +428,13 @@ GemmTile tuned_tile(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N,
   hipEvent_t e0, e1;
   DMRG_HIP(hipEventCreate(&e0));
   DMRG_HIP(hipEventCreate(&e1));
-  constexpr int NC = 4, ROUNDS = 4;
-  const GemmTile cand[NC] = {{32, 32, 1}, {32, 32, 2}, {64, 64, 1}, {64, 64, 2}};
+  // Only tiles that sum over k in the caller's order are candidates, so the
+  // choice never changes a result: with one wave per output sub-tile every
+  // tile size accumulates k in the same sequence; an eight-wave (k-split)
+  // tile's order depends on its k-tile, so a k-split caller tile stands.
+  if (tl.ks != 1) return tl;
+  constexpr int NC = 2, ROUNDS = 4;
+  const GemmTile cand[NC] = {{32, 32, 1}, {64, 64, 1}};
   float t_min[NC];
   GemmTile best = tl;
   float best_ms = 1e30f;
@@ -461,11 +467,10 @@ GemmTile tuned_tile(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N,
       }
     if (g_tune_log)
       std::fprintf(stderr,
-                   "[gemm autotune] dt %d op %d%d%s M %d N %d K %d: 32x32 %.1f, 32x32/8w %.1f, "
-                   "64x64 %.1f, 64x64/8w %.1f us -> %dx%d%s\n",
+                   "[gemm autotune] dt %d op %d%d%s M %d N %d K %d: 32x32 %.1f, 64x64 %.1f us -> "
+                   "%dx%d\n",
                    key.dt, opA, opB, herm ? " herm" : "", M, N, K, 1e3f * t_min[0],
-                   1e3f * t_min[1], 1e3f * t_min[2], 1e3f * t_min[3], best.bm, best.bn,
-                   best.ks == 2 ? "/8w" : "");
+                   1e3f * t_min[1], best.bm, best.bn);
   } catch (...) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
