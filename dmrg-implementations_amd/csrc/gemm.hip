@@ -801,8 +801,8 @@ namespace {
 // the k-loop no longer reads), and thread (a', b) contracts its 4D values
 // with WW into the 4D outputs Y[a' + cL ss' + 4 cL b + 4 cL cR n], summing
 // over c = w + D ss in the order of k_rows_small.  X never reaches HBM.
-// theta may be FP64 for an FP32 product (TB = double, T = float): it is
-// rounded to FP32 on load, as a separate conversion would round it.
+// theta may be stored in another precision (TB): each element is converted
+// to T on load.
 template <class T, class TB, int D, int BK>
 __global__ __launch_bounds__(256) void k_heff2_x_mpo(int cL, int cR, const T* __restrict__ Lenv,
                                                      const TB* __restrict__ theta,
@@ -1024,7 +1024,6 @@ DMRG_INST(double)
 DMRG_INST(cdouble)
 DMRG_INST(float)
 #undef DMRG_INST
-template bool heff2_x_mpo<float, double>(hipStream_t, int, int, int, int, const float*,
-                                         const double*, const float*, float*);
+
 
 }  // namespace dmrg

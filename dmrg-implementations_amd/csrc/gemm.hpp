@@ -79,7 +79,7 @@ bool gemm_rows_small(hipStream_t st, int M, int N, int K, const T* A, View2 va, 
 // formed tile by tile on the MFMA and contracted with WW in LDS, so X never
 // reaches HBM; the same sums in the same order as the GEMM followed by
 // gemm_rows_small.  Returns false (and does nothing) outside that range.
-// TB = double with T = float: an FP64 theta rounded to FP32 on load.
+// (Instantiated for TB = T.)
 template <class T, class TB = T>
 bool heff2_x_mpo(hipStream_t st, int cL, int cR, int d, int D, const T* Lenv, const TB* theta,
                  const T* WW, T* Y);
