@@ -57,3 +57,34 @@ def test_batch_reports_chain_error(dmrg_amd):
     assert ei.value.code == 5
     ok.close()
     bad.close()
+
+
+def test_batch_streams_returned_to_engines(dmrg_amd):
+    """The batch lends each worker's stream to the chains it runs and hands
+    the engines their own streams back: continuing every chain on its own
+    after a batch of 3 sweeps, then a second batch of 2, gives each chain
+    exactly the energy of 5 + 2 sweeps run one chain at a time (the kernels
+    and their order are the same; only the streams differ)."""
+    chains = _chains()
+    ref = []
+    for i, (L, chi, h) in enumerate(chains):
+        e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=30)
+        e.set_mpo(models.random_field_heisenberg_mpo(L, h))
+        e.initialize_mps_random(seed=200 + i, scale=0.1)
+        e.run(3)
+        e.run(2)
+        ref.append(e.run(2))
+        e.close()
+    engines = []
+    for i, (L, chi, h) in enumerate(chains):
+        e = dmrg_amd.DMRG(L, 2, chi, 5, tol=0.0, krylov_max=30)
+        e.set_mpo(models.random_field_heisenberg_mpo(L, h))
+        e.initialize_mps_random(seed=200 + i, scale=0.1)
+        engines.append(e)
+    dmrg_amd.run_batch(engines, 3, threads=2)
+    mid = [e.run(2) for e in engines]  # on the engines' own streams again
+    E, _ = dmrg_amd.run_batch(engines, 2, threads=4)
+    for e in engines:
+        e.close()
+    assert all(np.isfinite(mid))
+    assert list(E) == ref, (list(E), ref)
