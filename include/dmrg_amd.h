@@ -104,10 +104,12 @@ typedef struct dmrg_options {
                             (the reference's per-bond graph cache,
                             dmrg2_gpu_impl.h:615-637); ignored with profile. */
   int autotune;          /* 1: turn on the process-wide GEMM tile autotuner:
-                            the first call of each GEMM shape times the 32x32
-                            and 64x64 MFMA tiles (four or eight waves each)
-                            and keeps the fastest for every later call of
-                            that shape in the process.
+                            the first call of each GEMM shape whose default
+                            tile has one wave per output sub-tile times the
+                            32x32 and 64x64 MFMA tiles and keeps the faster
+                            for every later call of that shape in the
+                            process.  Both sum over k in the same order, so
+                            results do not depend on the choice.
                             No reference counterpart (rocBLAS picks its own
                             kernels).                                         */
 } dmrg_options;
