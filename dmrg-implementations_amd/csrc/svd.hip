@@ -872,44 +872,46 @@ __global__ void k_copy_rows(const T* __restrict__ Y, int ldy, int nk, int n, T* 
 // E_ij = A_ij / (lambda_j - lambda_i) for pairs the perturbation resolves
 // (|lambda_j - lambda_i| > sep |A_ij|), 0 otherwise (left to Jacobi) and on
 // the diagonal.  E is exactly anti-Hermitian (A is Hermitian and the test is
-// symmetric), so I + E is unitary up to O(|E|^2).  One workgroup per column
-// c: colpart[c] = sum_i |E_ic|, colpart[Np + c] = sum_i |E_ic|^2.
+// symmetric), so I + E is unitary up to O(|E|^2).  One workgroup per column.
 template <class T>
 __global__ __launch_bounds__(256) void k_oa_q(const T* __restrict__ A, T* __restrict__ Q, int N,
-                                              int Np, double sep, double* __restrict__ colpart) {
-  __shared__ double red1[256], red2[256];
-  const int c = blockIdx.x, t = threadIdx.x;
+                                              int Np, double sep) {
+  const int c = blockIdx.x;
   const double lc = real(A[c + (size_t)c * Np]);
-  double s1 = 0.0, s2 = 0.0;
-  for (int r = t; r < Np; r += 256) {
+  for (int r = threadIdx.x; r < Np; r += 256) {
     const size_t e = r + (size_t)c * Np;
     T q = T(0);
     if (r != c && r < N && c < N) {
       const T a = A[e];
       const double g = lc - real(A[r + (size_t)r * Np]);
-      if (fabs(g) > sep * sqrt(abs2(a))) {
-        q = a * (1.0 / g);
-        const double q2 = abs2(q);
-        s1 += sqrt(q2);
-        s2 += q2;
-      }
+      if (fabs(g) > sep * sqrt(abs2(a))) q = a * (1.0 / g);
     }
     Q[e] = q;
   }
-  red1[t] = s1;
-  red2[t] = s2;
+}
+
+// First Newton-Schulz factor from F = E^2 (Q_0 = I + E, Q_0^H Q_0 = I - F):
+// F <- (3I - (I - F))/2 = I + F/2, and part[blk] = sum of |F|^2 =
+// |Q_0^H Q_0 - I|^2 over the workgroup's entries (grid NRED).
+template <class T>
+__global__ __launch_bounds__(256) void k_ns_first(T* __restrict__ F, int n, double* __restrict__ part) {
+  __shared__ double red[256];
+  const size_t tot = (size_t)n * n;
+  double s2 = 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    const T f = F[e];
+    s2 += abs2(f);
+    F[e] = f * 0.5 + T(r == c ? 1.0 : 0.0);
+  }
+  red[threadIdx.x] = s2;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) {
-      red1[t] += red1[t + o];
-      red2[t] += red2[t + o];
-    }
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (t == 0) {
-    colpart[c] = red1[0];
-    colpart[Np + c] = red2[0];
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // P <- (3I - P)/2 (one Newton-Schulz step's polynomial factor); with part
@@ -937,16 +939,6 @@ __global__ __launch_bounds__(256) void k_ns_half(T* __restrict__ P, int n, doubl
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// H <- a F + b I (n x n).
-template <class T>
-__global__ void k_axpy_eye(T* __restrict__ H, const T* __restrict__ F, int n, double a, double b) {
-  const size_t tot = (size_t)n * n;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % n), c = (int)(e / n);
-    H[e] = a * F[e] + T(r == c ? b : 0.0);
-  }
-}
 // X_ii += b.
 template <class T>
 __global__ void k_add_diag(T* __restrict__ X, int n, double b) {
@@ -1246,82 +1238,60 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
         if (keep > 0 && sep > 30.0) sep = 30.0;
         else break;                                  // Jacobi
       }
-      hipLaunchKernelGGL(k_oa_q<T>, dim3(Np), dim3(256), 0, st, (const T*)w.A[0], w.Y, N, Np, sep,
-                         w.colpart);
-      // The unitary factor of I + E, Q2 = (I + E)(I - E^2)^(-1/2) (E is
-      // anti-Hermitian: (I + E)^H (I + E) = I - E^2).  -E^2 is positive
-      // semi-definite with |E^2|_2 = |E|_2^2 <= b = min(|E|_F^2, |E|_1^2)
-      // (|E|_1 = |E|_inf, the largest column sum, bounds |E|_2 since |E| is
-      // symmetric).  With a small b, by the binomial series
-      // (I - E^2)^(-1/2) = sum_k c_k F^k, F = E^2, c_k = C(2k, k) / 4^k,
-      // summed by Horner's rule: the series alternates, so the error of the
-      // sum up to F^m is below c_{m+1} b^(m+1), and m is the smallest order
-      // that brings it under 5e-17 (m + 1 GEMMs; m = 1 for b <= 1e-8).
-      // Otherwise Newton-Schulz on I + E, Q <- Q (3I - Q^H Q)/2, which maps
-      // the deviation delta = |Q^H Q - I|_2 to (3/4) delta^2 + O(delta^3),
-      // with delta_0 <= b: the Frobenius norm of Q^H Q - I is read before
-      // steps 2 and 4, and the step after a reading <= 1e-8 is the last.
-      double* const cp = w.host;  // 2 Np column sums (host capacity 2 cap + 2 NRED)
-      DMRG_HIP(hipMemcpyAsync(cp, w.colpart, 2 * (size_t)Np * sizeof(double),
-                              hipMemcpyDeviceToHost, st));
-      DMRG_HIP(hipStreamSynchronize(st));
-      double e2 = 0.0, c1 = 0.0;
-      for (int c = 0; c < Np; ++c) {
-        c1 = std::max(c1, cp[c]);
-        e2 += cp[Np + c];
-      }
-      if (!(e2 == e2) || !(c1 == c1)) break;
-      const double bnd = std::min(e2, c1 * c1);
-      constexpr double cser[8] = {1.0, 0.5, 0.375, 0.3125, 0.2734375, 0.24609375, 0.2255859375,
-                                  0.20947265625};
-      int m = 1;
-      while (m < 6 && cser[m + 1] * std::pow(bnd, m + 1) > 5e-17) ++m;
+      hipLaunchKernelGGL(k_oa_q<T>, dim3(Np), dim3(256), 0, st, (const T*)w.A[0], w.Y, N, Np, sep);
+      // The unitary factor Q2 of I + E by Newton-Schulz, Q <- Q (3I - Q^H Q)/2
+      // from Q_0 = I + E.  Every eigenvalue x of Q^H Q - I goes to
+      // -x^2 (3 - x)/4, so the Frobenius deviation delta = |Q^H Q - I|_F obeys
+      // delta' <= delta^2 (3 + delta)/4.  E is anti-Hermitian, so
+      // Q_0^H Q_0 = I - F with F = E^2: the first step's factor is I + F/2
+      // (one product less), and delta_0 = |F|_F is read with it.  The
+      // recursion then fixes the step count: the step taken once the bound is
+      // <= 1e-8 is the last (it leaves O(1e-16)).  From a start that four steps
+      // do not certify, the reading before step 4 decides (<= 1e-8 to take
+      // it, else Jacobi takes over with A and V unchanged).
       T* freeb[4] = {w.A[1], w.V[1], w.W, w.Y};  // Y holds E
-      T* Q2 = nullptr;
-      if (cser[m + 1] * std::pow(bnd, m + 1) <= 5e-17) {
-        T* F = freeb[0];
-        T* H = freeb[1];
-        T* G = freeb[2];
-        gemm_herm<T>(st, OP_N, OP_N, Np, Np, T(1), w.Y, Np, w.Y, Np, F, Np, tl);  // F = E^2
-        hipLaunchKernelGGL(k_axpy_eye<T>, dim3(grid_for(tot)), dim3(256), 0, st, H, (const T*)F,
-                           Np, cser[m], cser[m - 1]);
-        for (int j = m - 2; j >= 0; --j) {
-          gemm_herm<T>(st, OP_N, OP_N, Np, Np, T(1), F, Np, H, Np, G, Np, tl);  // F, H commute
-          hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, G, Np, cser[j]);
-          std::swap(H, G);
-        }
-        hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, w.Y, Np, 1.0);
-        DMRG_LAUNCH_CHECK();
-        Q2 = G;
-        gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), w.Y, Np, H, Np, T(0), Q2, Np, tl);  // (I + E) S
-      } else {
-        hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, w.Y, Np, 1.0);
-        DMRG_LAUNCH_CHECK();
-        double* const nspart = w.red + 2 * NRED;  // free while no offmax is pending
-        T* Z = w.Y;
-        T* P = freeb[0];
-        T* Zn = freeb[1];
-        bool unitary = false;
-        for (int ns = 0; ns < 4 && !unitary; ++ns) {
-          const bool check = ns == 1 || ns == 3;
-          gemm_herm<T>(st, OP_H, OP_N, Np, Np, T(1), Z, Np, Z, Np, P, Np, tl);
-          hipLaunchKernelGGL(k_ns_half<T>, dim3(NRED), dim3(256), 0, st, P, Np,
-                             check ? nspart : nullptr);
-          gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), Z, Np, P, Np, T(0), Zn, Np, tl);
-          std::swap(Z, Zn);
-          if (check) {
-            double* const dp = w.host;
-            DMRG_HIP(hipMemcpyAsync(dp, nspart, NRED * sizeof(double), hipMemcpyDeviceToHost, st));
-            DMRG_HIP(hipStreamSynchronize(st));
-            double d2 = 0.0;
-            for (int b = 0; b < NRED; ++b) d2 += dp[b];
-            if (!(d2 == d2)) break;
-            unitary = d2 <= 1e-16;  // |Q^H Q - I|_F <= 1e-8 before this step
-          }
-        }
-        if (!unitary) break;  // leave it to Jacobi (A, V unchanged)
-        Q2 = Z;
+      double* const nspart = w.red + 2 * NRED;  // free while no offmax is pending
+      T* Z = w.Y;
+      T* P = freeb[0];
+      T* Zn = freeb[1];
+      gemm_herm<T>(st, OP_N, OP_N, Np, Np, T(1), w.Y, Np, w.Y, Np, P, Np, tl);  // F = E^2
+      hipLaunchKernelGGL(k_ns_first<T>, dim3(NRED), dim3(256), 0, st, P, Np, nspart);
+      hipLaunchKernelGGL(k_add_diag<T>, dim3(cdiv(Np, 256)), dim3(256), 0, st, w.Y, Np, 1.0);
+      DMRG_LAUNCH_CHECK();
+      gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), Z, Np, P, Np, T(0), Zn, Np, tl);
+      std::swap(Z, Zn);
+      double* const dp = w.host;
+      DMRG_HIP(hipMemcpyAsync(dp, nspart, NRED * sizeof(double), hipMemcpyDeviceToHost, st));
+      DMRG_HIP(hipStreamSynchronize(st));
+      double d2 = 0.0;
+      for (int b = 0; b < NRED; ++b) d2 += dp[b];
+      if (!(d2 == d2)) break;  // non-finite generator: Jacobi
+      double dl = std::sqrt(d2);  // bound on the deviation before step `last`
+      int last = 0;
+      while (last < 3 && dl > 1e-8) {
+        dl = dl * dl * (3.0 + dl) * 0.25;
+        ++last;
       }
+      const bool certified = dl <= 1e-8;
+      if (!certified) last = 3;
+      bool unitary = certified;
+      for (int ns = 1; ns <= last; ++ns) {
+        const bool check = !certified && ns == 3;
+        gemm_herm<T>(st, OP_H, OP_N, Np, Np, T(1), Z, Np, Z, Np, P, Np, tl);
+        hipLaunchKernelGGL(k_ns_half<T>, dim3(NRED), dim3(256), 0, st, P, Np,
+                           check ? nspart : nullptr);
+        gemm<T>(st, OP_N, OP_N, Np, Np, Np, T(1), Z, Np, P, Np, T(0), Zn, Np, tl);
+        std::swap(Z, Zn);
+        if (check) {
+          DMRG_HIP(hipMemcpyAsync(dp, nspart, NRED * sizeof(double), hipMemcpyDeviceToHost, st));
+          DMRG_HIP(hipStreamSynchronize(st));
+          double c2 = 0.0;
+          for (int b = 0; b < NRED; ++b) c2 += dp[b];
+          unitary = c2 <= 1e-16;  // |Q^H Q - I|_F <= 1e-8 before this step (false for NaN)
+        }
+      }
+      if (!unitary) break;  // leave it to Jacobi (A, V unchanged)
+      T* const Q2 = Z;
       // The three other work buffers: X1 = A Q2, X2 = Q2^H X1 (the new A),
       // X3 = V Q2 (the new V).
       T* xs[3];
@@ -1407,7 +1377,6 @@ void SvdWork<T>::alloc(int cap_n) {
   DMRG_HIP(hipMalloc(&rank, cap * sizeof(int)));
   DMRG_HIP(hipMalloc(&gdiag, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&red, 4 * NRED * sizeof(double)));
-  DMRG_HIP(hipMalloc(&colpart, 2 * (size_t)cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&sigma, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&resid, cap * sizeof(double)));
   DMRG_HIP(hipMalloc(&scal, 4 * sizeof(double)));
@@ -1426,7 +1395,7 @@ void SvdWork<T>::release() {
     if (V[i]) (void)hipFree(V[i]);
     A[i] = V[i] = nullptr;
   }
-  void* ps[] = {Y, W, Q, qflag, perm, rank, gdiag, red, colpart, sigma, resid, scal, newk};
+  void* ps[] = {Y, W, Q, qflag, perm, rank, gdiag, red, sigma, resid, scal, newk};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (host) (void)hipHostFree(host);
@@ -1441,7 +1410,6 @@ void SvdWork<T>::release() {
   rank = nullptr;
   gdiag = nullptr;
   red = nullptr;
-  colpart = nullptr;
   sigma = nullptr;
   resid = nullptr;
   scal = nullptr;

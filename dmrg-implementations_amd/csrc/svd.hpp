@@ -48,7 +48,6 @@ struct SvdWork {
   int* rank = nullptr;           // inverse of perm (position of each index)
   double* gdiag = nullptr;       // squared norms of the k product vectors
   double* red = nullptr;         // reduction partials / small results (1024 doubles)
-  double* colpart = nullptr;     // per-column sums of the first-order step (2 cap doubles)
   double* sigma = nullptr;       // singular values in output order
   double* resid = nullptr;       // first-pass QR residuals
   double* scal = nullptr;        // [0] |rho|_F, [1] discarded weight, [2] trace(rho)
