@@ -406,6 +406,11 @@ template <class T> constexpr int dt_code() { return is_cplx<T>::value ? 1 : (std
 template <class T>
 GemmTile tuned_tile(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N, int K,
                     const T* A, int lda, const T* B, int ldb, int herm) {
+  // Only tiles that sum over k in the caller's order are candidates, so the
+  // choice never changes a result: with one wave per output sub-tile every
+  // tile size accumulates k in the same sequence; an eight-wave (k-split)
+  // tile's order depends on its k-tile, so a k-split caller tile stands.
+  if (tl.ks != 1) return tl;
   const TuneKey key{dt_code<T>(), opA, opB, M, N, K, herm};
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(key);
@@ -428,11 +433,6 @@ GemmTile tuned_tile(hipStream_t st, int opA, int opB, GemmTile tl, int M, int N,
   hipEvent_t e0, e1;
   DMRG_HIP(hipEventCreate(&e0));
   DMRG_HIP(hipEventCreate(&e1));
-  // Only tiles that sum over k in the caller's order are candidates, so the
-  // choice never changes a result: with one wave per output sub-tile every
-  // tile size accumulates k in the same sequence; an eight-wave (k-split)
-  // tile's order depends on its k-tile, so a k-split caller tile stands.
-  if (tl.ks != 1) return tl;
   constexpr int NC = 2, ROUNDS = 4;
   const GemmTile cand[NC] = {{32, 32, 1}, {64, 64, 1}};
   float t_min[NC];
