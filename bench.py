@@ -14,9 +14,10 @@ exactly K timed sweeps, one per engine call (the environment build before each
 call is outside the sweep timer, as in the reference).  The defaults, W=3 and
 K=9, make the 12-sweep C4 run.  `value` is the mixed path's mean seconds per
 timed sweep; every sweep's time (warm-up ones included), their mean over the
-whole run and the median of the timed ones are in `modes`.  The roofline entry is the H_eff matvec (two MFMA GEMM launches
-around the MPO-bond row kernel), timed with HIP events on the engine's
-stream.  |dE| compares the two paths after the same number of sweeps from the
+whole run and the median of the timed ones are in `modes`.  The roofline entry is the H_eff matvec (the first step fused with the
+MPO-bond step, the split-K out-step slices and their sum; in the mixed modes
+also the FP64/FP32 conversion of the two-site vector), timed with HIP events
+on the engine's stream.  |dE| compares the two paths after the same number of sweeps from the
 same state.
 
 Other workloads (not the headline line): c2 (Heisenberg L=64 chi=256), c3
