@@ -13,6 +13,12 @@
 namespace dmrg {
 namespace {
 
+// Row and column of flat index e of a column-major matrix with n rows, in
+// 32-bit arithmetic (every matrix these kernels index has at most 2048^2
+// entries; a 64-bit division is a long software sequence on the GPU).
+__device__ inline int mod32(size_t e, int n) { return (int)((unsigned)e % (unsigned)n); }
+__device__ inline int div32(size_t e, int n) { return (int)((unsigned)e / (unsigned)n); }
+
 constexpr int JB = 16;       // Jacobi block width
 constexpr int JP = 2 * JB;   // order of one pair sub-problem
 constexpr int LDP = JP + 1;  // LDS column stride (padding)
@@ -87,7 +93,7 @@ __global__ void k_prep(T* __restrict__ A, T* __restrict__ V, int N, int Np, int 
   const size_t tot = (size_t)Np * Np;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % Np), c = (int)(e / Np);
+    const int r = mod32(e, Np), c = div32(e, Np);
     if (r >= N || c >= N) A[e] = T(0);
     else if (r > c) A[e] = conj(A[c + (size_t)r * Np]);
     else if (r == c) A[e] = T(real(A[e]));
@@ -101,7 +107,7 @@ __global__ void k_set_basis(T* __restrict__ V, const T* __restrict__ X, int N, i
   const size_t tot = (size_t)Np * Np;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % Np), c = (int)(e / Np);
+    const int r = mod32(e, Np), c = div32(e, Np);
     V[e] = (r < N && c < N) ? X[r + (size_t)c * N] : ((r == c) ? T(1) : T(0));
   }
 }
@@ -146,6 +152,8 @@ __global__ __launch_bounds__(256) void k_norms_fin(const double* __restrict__ re
 // rank[i] = position of diagonal entry i in the stable descending order of
 // k_sort (ties by index, NaN as -inf); padding i >= N gets rank i.
 // keys[0] = largest entry, keys[1] = entry of rank keep-1.
+// Sixteen lanes per entry (all with the same i): lane q counts the entries
+// j = q, q + 16, ...; the group sums by shuffles.  Grid: cdiv(Np, 16).
 template <class T>
 __global__ __launch_bounds__(256) void k_rank_diag(const T* __restrict__ A, int N, int Np,
                                                    int keep, int* __restrict__ rank,
@@ -157,22 +165,22 @@ __global__ __launch_bounds__(256) void k_rank_diag(const T* __restrict__ A, int 
     dg[i] = (v == v) ? v : -INFINITY;
   }
   __syncthreads();
-  // Four lanes per entry (a quad, all with the same i): lane q counts the
-  // entries j = q, q + 4, ...; the quad sums by shuffles.  Grid: cdiv(Np, 64).
-  const int q = t & 3;
-  for (int i = blockIdx.x * 64 + (t >> 2); i < Np; i += gridDim.x * 64) {
+  const int q = t & 15;
+  for (int i = blockIdx.x * 16 + (t >> 4); i < Np; i += gridDim.x * 16) {
     int r = 0;
     double di = 0.0;
     if (i < N) {
       di = dg[i];
 #pragma unroll 8
-      for (int j = q; j < N; j += 4) {
+      for (int j = q; j < N; j += 16) {
         const double dj = dg[j];
         r += (dj > di) || (dj == di && j < i);
       }
     }
     r += __shfl_xor(r, 1);
     r += __shfl_xor(r, 2);
+    r += __shfl_xor(r, 4);
+    r += __shfl_xor(r, 8);
     if (q == 0) {
       if (i >= N) {
         rank[i] = i;
@@ -235,7 +243,7 @@ __global__ void k_permute(const T* __restrict__ A, const T* __restrict__ V, T* _
   const size_t tot = (size_t)Np * Np;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % Np), c = (int)(e / Np);
+    const int r = mod32(e, Np), c = div32(e, Np);
     const int pc = perm[c];
     B[e] = A[perm[r] + (size_t)pc * Np];
     W[e] = V[r + (size_t)pc * Np];
@@ -685,24 +693,31 @@ __global__ __launch_bounds__(NT) void k_pair_apply(T* __restrict__ A, T* __restr
 }
 
 
-// part[blk] = max |A(i,j)|, i != j, over the workgroup's entries of the N x N
-// block (NaN as +inf); grid of NRED.  With rank given, only pairs with exactly
-// one of rank[i], rank[j] < keep.
+// part[blk] = max |A(i,j)|, i != j, over the workgroup's columns j of the
+// N x N block (NaN as +inf); grid of NRED, columns blk, blk + NRED, ...  With
+// rank given, only pairs with exactly one of rank[i], rank[j] < keep (the
+// side of every index is tabulated in LDS first).
 template <class T>
 __global__ __launch_bounds__(256) void k_offmax(const T* __restrict__ A, int N, int Np,
                                                 const int* __restrict__ rank, int keep,
                                                 double* __restrict__ part) {
   __shared__ double red[256];
+  __shared__ unsigned char side[SORT_MAX];
   const int t = threadIdx.x;
+  if (rank) {
+    for (int i = t; i < N; i += 256) side[i] = rank[i] < keep ? 1 : 0;
+    __syncthreads();
+  }
   double mx = 0.0;
-  const size_t tot = (size_t)Np * Np;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + t; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % Np), c = (int)(e / Np);
-    if (r == c || r >= N || c >= N) continue;
-    if (rank && ((rank[r] < keep) == (rank[c] < keep))) continue;
-    double v = absv(A[e]);
-    if (!(v == v)) v = INFINITY;
-    mx = fmax(mx, v);
+  for (int c = blockIdx.x; c < N; c += gridDim.x) {
+    const T* col = A + (size_t)c * Np;
+    const unsigned char sc = rank ? side[c] : 0;
+    for (int r = t; r < N; r += 256) {
+      if (r == c || (rank && side[r] == sc)) continue;
+      double v = absv(col[r]);
+      if (!(v == v)) v = INFINITY;
+      mx = fmax(mx, v);
+    }
   }
   red[t] = mx;
   __syncthreads();
@@ -721,10 +736,10 @@ __global__ void k_gather(const T* __restrict__ V, int ldv, const int* __restrict
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
     if (!herm) {
-      const int i = (int)(e % rows), j = (int)(e / rows);
+      const int i = mod32(e, rows), j = div32(e, rows);
       out[i + (size_t)j * ldo] = V[i + (size_t)perm[j] * ldv];
     } else {
-      const int j = (int)(e % cols), i = (int)(e / cols);
+      const int j = mod32(e, cols), i = div32(e, cols);
       out[j + (size_t)i * ldo] = conj(V[i + (size_t)perm[j] * ldv]);
     }
   }
@@ -778,8 +793,8 @@ __global__ void k_clean_scale(const T* __restrict__ Y, int ldy, int k, int len, 
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
     int i, c;
-    if (by_row) { i = (int)(e % k); c = (int)(e / k); }
-    else { c = (int)(e % len); i = (int)(e / len); }
+    if (by_row) { i = mod32(e, k); c = div32(e, k); }
+    else { c = mod32(e, len); i = div32(e, len); }
     const double gii = real(G[i + (size_t)i * ldg]);
     const bool clean = gii > 0.0 && r0[i] >= 0.5 * gii;
     const double f = clean ? 1.0 / sqrt(gii) : 0.0;
@@ -795,7 +810,7 @@ __global__ void k_mask_tri(T* __restrict__ C, int k, int keep_lower) {
   const size_t tot = (size_t)k * k;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % k), c = (int)(e / k);
+    const int r = mod32(e, k), c = div32(e, k);
     const bool keep = keep_lower ? (r > c) : (r < c);
     if (!keep) C[e] = T(0);
   }
@@ -862,7 +877,7 @@ __global__ void k_copy_rows(const T* __restrict__ Y, int ldy, int nk, int n, T* 
   const size_t tot = (size_t)nk * n;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int j = (int)(e % nk), c = (int)(e / nk);
+    const int j = mod32(e, nk), c = div32(e, nk);
     out[e] = Y[j + (size_t)c * ldy];
   }
 }
@@ -900,7 +915,7 @@ __global__ __launch_bounds__(256) void k_ns_first(T* __restrict__ F, int n, doub
   double s2 = 0.0;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % n), c = (int)(e / n);
+    const int r = mod32(e, n), c = div32(e, n);
     const T f = F[e];
     s2 += abs2(f);
     F[e] = f * 0.5 + T(r == c ? 1.0 : 0.0);
@@ -923,7 +938,7 @@ __global__ __launch_bounds__(256) void k_ns_half(T* __restrict__ P, int n, doubl
   double s2 = 0.0;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % n), c = (int)(e / n);
+    const int r = mod32(e, n), c = div32(e, n);
     const T one = (r == c) ? T(1) : T(0);
     const T p = P[e];
     s2 += abs2(p - one);
@@ -1182,7 +1197,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
   double* const offpart = w.red + 2 * NRED;  // per-workgroup maxima of k_offmax
   auto offmax = [&](const T* M) {
     if (keep > 0)
-      hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 64)), dim3(256), 0, st, M, N, Np, keep,
+      hipLaunchKernelGGL(k_rank_diag<T>, dim3(cdiv(Np, 16)), dim3(256), 0, st, M, N, Np, keep,
                          w.rank, w.red);
     hipLaunchKernelGGL(k_offmax<T>, dim3(NRED), dim3(256), 0, st, M, N, Np,
                        keep > 0 ? (const int*)w.rank : nullptr, keep, offpart);

@@ -9,6 +9,12 @@
 namespace dmrg {
 namespace {
 
+// Row and column of flat index e of a column-major matrix with n rows, in
+// 32-bit arithmetic (every matrix these kernels index has at most 2048^2
+// entries; a 64-bit division is a long software sequence on the GPU).
+__device__ inline int mod32(size_t e, int n) { return (int)((unsigned)e % (unsigned)n); }
+__device__ inline int div32(size_t e, int n) { return (int)((unsigned)e / (unsigned)n); }
+
 __device__ inline double absv(double a) { return fabs(a); }
 __device__ inline double absv(cdouble a) { return sqrt(a.x * a.x + a.y * a.y); }
 
@@ -79,7 +85,7 @@ __global__ __launch_bounds__(256) void k_ns_prep(T* __restrict__ P, int n, doubl
   double mx = 0.0;
   const size_t tot = (size_t)n * n;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + t; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e % n), c = (int)(e / n);
+    const int r = mod32(e, n), c = div32(e, n);
     const T one = (r == c) ? T(1) : T(0);
     const T p = P[e];
     double v = absv(p - one);
