@@ -194,10 +194,18 @@ __global__ void k_zero_d(double* x) { *x = 0.0; }
 __global__ __launch_bounds__(256) void k_absmax(const double* in, size_t n, unsigned long long* out) {
   __shared__ double red[256];
   const int t = threadIdx.x;
-  double m = 0.0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + t; i < n; i += (size_t)gridDim.x * blockDim.x)
-    m = fmax(m, fabs(in[i]));
-  red[t] = m;
+  // four independent maxima keep four loads in flight per thread
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  size_t i = blockIdx.x * (size_t)blockDim.x + t;
+  for (; i + 3 * st < n; i += 4 * st) {
+    m0 = fmax(m0, fabs(in[i]));
+    m1 = fmax(m1, fabs(in[i + st]));
+    m2 = fmax(m2, fabs(in[i + 2 * st]));
+    m3 = fmax(m3, fabs(in[i + 3 * st]));
+  }
+  for (; i < n; i += st) m0 = fmax(m0, fabs(in[i]));
+  red[t] = fmax(fmax(m0, m1), fmax(m2, m3));
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (t < o) red[t] = fmax(red[t], red[t + o]);
@@ -257,15 +265,13 @@ __global__ void k_fill_d(double* s, size_t n, double v) {
        i += (size_t)gridDim.x * blockDim.x)
     s[i] = v;
 }
+// Grid (x: row blocks of 256, y: columns, strided by gridDim.y).
 template <class T>
 __global__ void k_copy2d(T* __restrict__ dst, size_t ldd, const T* __restrict__ src, size_t lds,
                          size_t rows, size_t cols) {
-  const size_t tot = rows * cols;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const size_t i = e % rows, j = e / rows;
-    dst[i + j * ldd] = src[i + j * lds];
-  }
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  for (size_t j = blockIdx.y; j < cols; j += gridDim.y) dst[i + j * ldd] = src[i + j * lds];
 }
 template <class T>
 __global__ void k_unit(T* env, int D, int w) {
@@ -289,7 +295,8 @@ void convert_f32_f64(hipStream_t st, const float* in, double* out, size_t n) {
 void absmax_f64(hipStream_t st, const double* in, size_t n, double* out) {
   hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, st, out);
   if (n)
-    hipLaunchKernelGGL(k_absmax, dim3(std::min<size_t>(cdiv(n, 256), 1024)), dim3(256), 0, st, in,
+    // at most 256 workgroups: each ends in one atomic on the same word
+    hipLaunchKernelGGL(k_absmax, dim3(std::min<size_t>(cdiv(n, 256), 256)), dim3(256), 0, st, in,
                        n, reinterpret_cast<unsigned long long*>(out));
   DMRG_LAUNCH_CHECK();
 }
@@ -326,8 +333,9 @@ void copy_dev(hipStream_t st, T* dst, size_t ldd, const T* src, size_t lds, size
               size_t cols) {
   const size_t tot = rows * cols;
   if (!tot) return;
-  hipLaunchKernelGGL(k_copy2d<T>, dim3(std::min<size_t>(cdiv(tot, 256), 8192)), dim3(256), 0, st,
-                     dst, ldd, src, lds, rows, cols);
+  const unsigned gx = cdiv(rows, 256);
+  const unsigned gy = (unsigned)std::min<size_t>(cols, std::max<size_t>(1, 8192 / gx));
+  hipLaunchKernelGGL(k_copy2d<T>, dim3(gx, gy), dim3(256), 0, st, dst, ldd, src, lds, rows, cols);
   DMRG_LAUNCH_CHECK();
 }
 template void copy_dev<double>(hipStream_t, double*, size_t, const double*, size_t, size_t, size_t);

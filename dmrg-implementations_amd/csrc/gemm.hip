@@ -741,12 +741,14 @@ __global__ void k_splitk_sum(const T* __restrict__ W, int S, long long stride, i
     f = (a > 0.0 ? a : 1.0) * (b > 0.0 ? b : 1.0);
   }
   const long long tot = (long long)M * N;
+  const bool packed = ldc == M;  // C contiguous: no row/column split per element
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
        e += (long long)gridDim.x * blockDim.x) {
     T v = W[e];
     for (int z = 1; z < S; ++z) v = v + W[e + z * stride];
-    if constexpr (std::is_same<T, TO>::value) C[(e % M) + (e / M) * (long long)ldc] = v;
-    else C[(e % M) + (e / M) * (long long)ldc] = (TO)v * f;
+    const long long o = packed ? e : (e % M) + (e / M) * (long long)ldc;
+    if constexpr (std::is_same<T, TO>::value) C[o] = v;
+    else C[o] = (TO)v * f;
   }
 }
 

@@ -197,42 +197,39 @@ __global__ __launch_bounds__(256) void k_rank_diag(const T* __restrict__ A, int 
 // Key i is the real diagonal A(i, i) (ld Np) when A is given, else keys[i];
 // positions i >= n_real (zero padding of the Jacobi matrix) get -inf, so the
 // padding always sorts to the end and real indices keep positions < n_real.
+// By ranks: the position of key i is the number of keys before it in the
+// order (larger, or equal with a smaller index; NaN as -inf), and
+// perm[position] = i.  Sixteen lanes per key count j = q, q + 16, ... and sum
+// by shuffles; grid cdiv(n, 16) workgroups of 256 threads.
 template <class T>
-__global__ __launch_bounds__(1024) void k_sort(const T* __restrict__ A, int Np,
-                                               const double* __restrict__ keys, int n, int n_real,
-                                               int* __restrict__ perm, double* __restrict__ keys_out) {
+__global__ __launch_bounds__(256) void k_sort(const T* __restrict__ A, int Np,
+                                              const double* __restrict__ keys, int n, int n_real,
+                                              int* __restrict__ perm, double* __restrict__ keys_out) {
   __shared__ double key[SORT_MAX];
-  __shared__ int idx[SORT_MAX];
   const int t = threadIdx.x;
-  int sz = 64;  // bitonic network size: the next power of two >= n
-  while (sz < n) sz <<= 1;
-  for (int i = t; i < sz; i += 1024) {
+  for (int i = t; i < n; i += 256) {
     double v = -INFINITY;
     if (i < n_real) v = A ? real(A[i + (size_t)i * Np]) : keys[i];
-    if (!(v == v)) v = -INFINITY;  // keeps every output index below n
-    key[i] = v;
-    idx[i] = i;
+    key[i] = (v == v) ? v : -INFINITY;  // keeps every output index below n
   }
   __syncthreads();
-  for (int size = 2; size <= sz; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = t; i < sz; i += 1024) {
-        const int j = i ^ stride;
-        if (j > i) {
-          const bool desc = ((i & size) == 0);
-          const bool ib = (key[i] > key[j]) || (key[i] == key[j] && idx[i] < idx[j]);
-          if (desc ? !ib : ib) {
-            double tk = key[i]; key[i] = key[j]; key[j] = tk;
-            int ti = idx[i]; idx[i] = idx[j]; idx[j] = ti;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  const int q = t & 15;
+  const int i = blockIdx.x * 16 + (t >> 4);
+  if (i >= n) return;  // whole groups of sixteen lanes leave together
+  const double ki = key[i];
+  int r = 0;
+#pragma unroll 8
+  for (int j = q; j < n; j += 16) {
+    const double kj = key[j];
+    r += (kj > ki) || (kj == ki && j < i);
   }
-  for (int i = t; i < n; i += 1024) {
-    perm[i] = idx[i];
-    if (keys_out) keys_out[i] = key[i];
+  r += __shfl_xor(r, 1);
+  r += __shfl_xor(r, 2);
+  r += __shfl_xor(r, 4);
+  r += __shfl_xor(r, 8);
+  if (q == 0) {
+    perm[r] = i;
+    if (keys_out) keys_out[r] = ki;
   }
 }
 
@@ -1188,7 +1185,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
   sweeps = 0;
   off_rel = 0.0;
   if (fro == 0.0) {
-    hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N,
+    hipLaunchKernelGGL(k_sort<T>, dim3(cdiv(Np, 16)), dim3(256), 0, st, w.A[cur], Np, nullptr, Np, N,
                        w.perm, nullptr);
     DMRG_LAUNCH_CHECK();
     return cur;
@@ -1333,7 +1330,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
   int n_bip = 0;
   for (int sw = 0; sw < prm.max_sweeps && !done; ++sw) {
     // sort the diagonal, permute rho and V into the other buffer
-    hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N,
+    hipLaunchKernelGGL(k_sort<T>, dim3(cdiv(Np, 16)), dim3(256), 0, st, w.A[cur], Np, nullptr, Np, N,
                        w.perm, nullptr);
     hipLaunchKernelGGL(k_permute<T>, dim3(grid_for(tot)), dim3(256), 0, st, w.A[cur], w.V[cur],
                        w.A[1 - cur], w.V[1 - cur], w.perm, Np);
@@ -1366,7 +1363,7 @@ int jacobi(hipStream_t st, SvdWork<T>& w, int N, int Np, const SvdParams& prm, J
     if (off <= prm.tol * fro) { done = true; break; }
   }
   if (!done) throw Error(ST_SOLVER_NO_CONV, "svd: Jacobi eigen-solver did not converge");
-  hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, w.A[cur], Np, nullptr, Np, N, w.perm,
+  hipLaunchKernelGGL(k_sort<T>, dim3(cdiv(Np, 16)), dim3(256), 0, st, w.A[cur], Np, nullptr, Np, N, w.perm,
                      nullptr);
   DMRG_LAUNCH_CHECK();
   return cur;
@@ -1577,7 +1574,7 @@ int heig(hipStream_t st, SvdWork<T>& w, const T* H, int N, int ldh, double* eval
   double off = 0.0;
   const int cur = jacobi<T>(st, w, N, Np, prm, jc, off);
   const int sweeps = jc.sweeps;
-  hipLaunchKernelGGL(k_sort<T>, dim3(1), dim3(1024), 0, st, (const T*)w.A[cur], Np, nullptr, N, N,
+  hipLaunchKernelGGL(k_sort<T>, dim3(cdiv(N, 16)), dim3(256), 0, st, (const T*)w.A[cur], Np, nullptr, N, N,
                      w.perm, evals);
   hipLaunchKernelGGL(k_gather<T>, dim3(grid_for((size_t)N * N)), dim3(256), 0, st,
                      (const T*)w.V[cur], Np, (const int*)w.perm, N, N, 0, evecs, lde);
